@@ -1,0 +1,204 @@
+"""Benchmark of the hybrid DG residual + SSP-RK3 stage on one MI355X.
+
+Workload (the metric's configuration that fits one GPU): Taylor-Green vortex
+on a periodic [0, 2 pi]^3 box of 32^3 hexahedra, DG p=3 (64 nodes/element,
+2,097,152 DOF per variable), 5 conserved variables, gamma 1.4, Pr 0.71,
+Mach 0.1, Re 1600, HLLC + BR2, SSP-RK3 at CFL 0.3, no limiter, no chemistry.
+Synthetic data: the analytic TGV initial condition at the solution nodes.
+
+A timed step is one full SSP-RK3 step (three fused residual + RK-epilogue
+stages) on state already resident in HBM.  The plain FP64 path and the FP32
+path run the same number of steps from the same initial state with the same
+dt; ``value`` is the DOF-updates/s per RK stage of the FP32 path when its
+relative L2 error against the FP64 path stays within 1e-4 for density and
+energy, else of the FP64 path.  ``roofline`` reports the element kernel (the
+dominant launch) timed with HIP events on its own stream; ``cpu_baseline``
+times the CPU FP64 oracle on a bounded sample (8^3 elements, 2 stages).
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "jax-fluid-sim_amd"))
+
+PEAK = {"f64": 78.6, "f32": 157.3}          # TFLOP/s, MI355X dense
+HBM_PEAK = 8000.0                            # GB/s
+GAMMA, MACH, RE, PR = 1.4, 0.1, 1600.0, 0.71
+
+
+def tgv_primitive(x):
+    """TGV at Mach 0.1 (rho = 1, U = 1, p0 = 1/(gamma M^2))."""
+    X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+    p = 1.0 / (GAMMA * MACH ** 2) + (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2) / 16
+    return np.stack([np.ones_like(X), np.sin(X) * np.cos(Y) * np.cos(Z),
+                     -np.cos(X) * np.sin(Y) * np.cos(Z), np.zeros_like(X), p], -1)
+
+
+def element_flops(N, Q, V):
+    """Algorithmic flops of one element in the element kernel (dense operator
+    formulation): interpolation 2QNV, raw viscous gradient 2QN*12 + 2Q*36,
+    BR2 lift M^-1 2N^2*12 and interpolation 2QN*12, weak divergence
+    2N*3Q*V, inverse mass 2N^2 V."""
+    return (2 * Q * N * V + 2 * Q * N * 12 + 2 * Q * 36 + 2 * N * N * 12 + 2 * Q * N * 12
+            + 2 * N * 3 * Q * V + 2 * N * N * V)
+
+
+def run_path(topo, dtype, steps, warmup, dt):
+    import torch
+    from hybrid import HybridCompressibleSolver, HybridNavierStokesOperator, HybridSSPRK3
+    from hybrid import _native as nat
+    solver = HybridCompressibleSolver(topo, polynomial_degree=3, gamma=GAMMA,
+                                      dynamic_viscosity=1.0 / RE, prandtl=PR, dtype=dtype)
+    op = HybridNavierStokesOperator(solver)
+    rk = HybridSSPRK3(op, cfl=0.3)
+    U0 = solver.state_blocks_from_primitive_function(tgv_primitive)
+    if dt is None:
+        dt = float(rk.stable_dt(U0))
+    U = U0
+    for _ in range(warmup):
+        U, valid, _, _ = rk.checked_step(U, dt)
+    torch.cuda.synchronize()
+    ctx = op.device_mesh.ctx
+    nat.check(nat.lib().fsg_timing(ctx, 1))
+    U = {k: v.clone() for k, v in U0.items()}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        U, valid, rmin, pmin = rk.checked_step(U, dt)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    import ctypes
+    gms = (ctypes.c_double * len(op.device_mesh.groups))()
+    bms = (ctypes.c_double * len(op.device_mesh.block_tables))()
+    gn = (ctypes.c_int64 * len(op.device_mesh.groups))()
+    bn = (ctypes.c_int64 * len(op.device_mesh.block_tables))()
+    nat.check(nat.lib().fsg_timing_read(ctx, ctypes.addressof(gms), ctypes.addressof(bms),
+                                        ctypes.addressof(gn), ctypes.addressof(bn)))
+    nat.check(nat.lib().fsg_timing(ctx, 0))
+    return dict(state=U, elapsed=elapsed, dt=dt, valid=bool(valid), rho_min=float(rmin),
+                p_min=float(pmin), elem_ms=bms[0] / max(bn[0], 1), elem_launches=bn[0],
+                face_ms=gms[0] / max(gn[0], 1), face_launches=gn[0])
+
+
+def cpu_baseline():
+    """CPU FP64 oracle, one thread, on 8^3 elements for 2 RK stages."""
+    code = r"""
+import sys, time, json
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import numpy as np
+from bench import tgv_primitive, GAMMA, RE, PR
+from hybrid.meshgen import box_mesh
+from oracle.mesh import from_arrays
+from oracle.solver import CompressibleSolver
+from oracle.operators import NavierStokesOperator
+from oracle.time_integration import SSPRK3
+m = box_mesh(8, lengths=(2 * np.pi,) * 3, periodic=(True,) * 3)
+s = CompressibleSolver(gamma=GAMMA, dynamic_viscosity=1.0 / RE, prandtl=PR)
+op = NavierStokesOperator(from_arrays(m), s, 3)
+st = {k: s.primitive_to_conserved(tgv_primitive(x)) for k, x in op.geometry.solution_node_coordinates.items()}
+rk = SSPRK3(op, cfl=0.3)
+dt = rk.stable_dt(st)
+t0 = time.perf_counter()
+s1 = rk.rk_stage(st, st, dt, 0.0, 1.0)[0]
+s2 = rk.rk_stage(st, s1, dt, 0.75, 0.25)[0]
+t = (time.perf_counter() - t0) / 2
+print(json.dumps({"dof": 512 * 64, "stage_s": t}))
+""" % (str(ROOT), str(ROOT / "jax-fluid-sim_amd"))
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                       cwd=str(ROOT), timeout=600)
+    if r.returncode != 0:
+        return {"value": None, "error": r.stderr[-400:]}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": d["dof"] / d["stage_s"], "unit": "DOF-updates/s per RK stage",
+            "cores": 1, "kind": "port",
+            "sample": "CPU FP64 oracle (numpy), TGV 8^3 hexahedra p=3 (32,768 DOF/var), "
+                      "2 SSP-RK3 stages, 1 thread; per-DOF rate"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=32, help="elements per box edge")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--only", choices=["f64", "f32"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo exchange is a design, not a build)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hybrid import Topology, box_mesh
+    topo = Topology(box_mesh(args.n, lengths=(2 * np.pi,) * 3, periodic=(True,) * 3))
+    E = topo.blocks[0].count
+    N, Q, V = 64, 125, 5
+    dof = E * N
+    print(f"[bench] TGV {args.n}^3 p=3: {E} elements, {dof} DOF/var", file=sys.stderr, flush=True)
+
+    res = {}
+    dt = None
+    for name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        if args.only and args.only != name:
+            continue
+        r = run_path(topo, dtype, args.steps, args.warmup, dt)
+        dt = r["dt"]
+        r["rate"] = dof * 3 * args.steps / r["elapsed"]
+        res[name] = r
+        print(f"[bench] {name}: {r['elapsed'] * 1e3 / args.steps:.3f} ms/step, "
+              f"{r['rate']:.4e} DOF/s/stage, element {r['elem_ms']:.3f} ms, "
+              f"face {r['face_ms']:.3f} ms, valid={r['valid']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
+    err = None
+    if "f64" in res and "f32" in res:
+        a = res["f32"]["state"]["hexahedron"].double()
+        b = res["f64"]["state"]["hexahedron"]
+        err = (torch.linalg.vector_norm((a - b).reshape(-1, V), dim=0)
+               / torch.linalg.vector_norm(b.reshape(-1, V), dim=0)).cpu().numpy()
+    within = err is not None and err[0] <= 1e-4 and err[4] <= 1e-4
+    key = "f32" if within else ("f64" if "f64" in res else "f32")
+    r = res[key]
+    flops = element_flops(N, Q, V) * E
+    achieved = flops / (r["elem_ms"] * 1e-3) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / f"element_kernel_traffic_{key}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": "DOF-updates/s per RK stage, plain FP64 vs FP32+surrogate chem, at fixed L2 error",
+        "value": r["rate"], "unit": "DOF-updates/s per RK stage", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["elapsed"] * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": key, "data": "synthetic",
+        "config": {"workload": f"Taylor-Green vortex, periodic {args.n}^3 hexahedra, DG p=3, "
+                               "HLLC + BR2, SSP-RK3, CFL 0.3, Mach 0.1, Re 1600, Pr 0.71",
+                   "elements": E, "dof_per_var": dof, "dt": dt,
+                   "dt_clipped": bool(dt >= 1e-3 or dt <= 1e-7)},
+        "fp64_dof_updates_per_s": res.get("f64", {}).get("rate"),
+        "fp32_dof_updates_per_s": res.get("f32", {}).get("rate"),
+        "fp32_rel_l2_vs_fp64": None if err is None else
+            dict(zip(["rho", "rho_u", "rho_v", "rho_w", "rho_E"], map(float, err))),
+        "roofline": {"kernel": f"element_kernel<hex,p3,V5,{key}>", "bound": "mfma",
+                     "achieved": achieved, "peak": PEAK[key], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[key], "traffic": traffic,
+                     "flops_per_launch": flops, "avg_launch_ms": r["elem_ms"]},
+        "face_kernel_avg_ms": r["face_ms"],
+    }
+    if not args.skip_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/* C ABI of the MI355X-native hybrid DG hot path (libfluid_gfx950.so).
+ *
+ * All array arguments are device pointers owned by the caller (torch tensors
+ * on ROCm today; XLA buffers when the entry points are registered as
+ * jax.ffi custom calls).  A context keeps raw pointers to the caller's
+ * immutable tables and never allocates state memory.  No entry point
+ * synchronises the stream; diagnostics are written to device memory.
+ * Every entry point returns 0 on success or a negative status; the text of
+ * the last error is available from fsg_last_error().
+ *
+ * Precision: dtype FSG_F64 runs every kernel in double (the plain path);
+ * FSG_F32 runs the same kernels in float (the FP32 path).  The state and
+ * table buffers must be of the context's dtype.
+ */
+#ifndef FLUID_GFX950_H
+#define FLUID_GFX950_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSG_F64 0
+#define FSG_F32 1
+
+#define FSG_MODE_RESIDUAL 0 /* weak-form residual before M^-1 */
+#define FSG_MODE_RHS 1      /* M^-1 residual */
+
+#define FSG_TERM_EULER 1
+#define FSG_TERM_VISCOUS 2
+
+/* Boundary flags per boundary face (the tag classes of
+ * hybrid/boundary.py:22-46). */
+#define FSG_BND_WALL 1
+#define FSG_BND_INLET 2
+#define FSG_BND_NOSLIP 4
+#define FSG_BND_SLIP 8
+
+typedef struct fsg_context fsg_context;
+
+/* Physics of HybridCompressibleSolver (hybrid/solver.py:20-53) and the BR2
+ * penalty of HybridNavierStokesOperator (hybrid/navier_stokes.py:10-16). */
+typedef struct {
+  double gamma;
+  double gas_constant;
+  double dynamic_viscosity;
+  double thermal_conductivity;
+  double br2_penalty;
+  int32_t inviscid_flux; /* 0 = hllc, 1 = rusanov */
+  int32_t nvars;         /* 5 + number of species (5 or 16) */
+  int32_t degree;        /* 1, 2 or 3 */
+  int32_t dtype;         /* FSG_F64 or FSG_F32 */
+} fsg_params;
+
+/* One element block (hybrid/mesh.py:122-139 + hybrid/geometry.py:27-36 for
+ * affine cells).  Reference tables are shared by the block; per-element
+ * metrics are J^-1 (9 per element, [r*3+d] = d xi_r / d x_d), the
+ * sliver-scaled inverse-mass factor min(1, V/1e-4)/|det J| and |det J|.  The
+ * CSR (rec_*) lists each element's face-side records in the face-output
+ * buffer. */
+typedef struct {
+  int32_t element_type; /* 0 hexahedron, 1 wedge, 2 tetra */
+  int32_t reserved;
+  int64_t count;
+  const void* Bt;    /* (N, Q) */
+  const void* Dt;    /* (N, 3, Q) */
+  const void* D;     /* (Q, 3, N) */
+  const void* wq;    /* (Q) */
+  const void* minv;  /* (N, N) reference inverse mass */
+  const void* jinv;  /* (E, 9) */
+  const void* scale; /* (E) */
+  const void* detw;  /* (E) */
+  const int64_t* rec_ptr; /* (E + 1) */
+  const int64_t* rec_off; /* (R) offsets in reals into face_out */
+  const int32_t* rec_sup; /* (R) offsets into the support-index array */
+  const int32_t* rec_nf;  /* (R) support size */
+} fsg_block_desc;
+
+/* One face group: (face type, owner block, neighbour block) as in
+ * hybrid/spatial.py:20-39, further split by sub-faces (a quad face of the
+ * owner cut into two triangles).  Trace tables hold, per distinct set of
+ * reference points: face-support basis (Qf, NF), lift operator
+ * (Phi M_ref^-1 restricted to the support, (Qf, NF)) and reference
+ * gradients (Qf, N, 3). */
+typedef struct {
+  int32_t face_type;      /* 0 quad, 1 triangle */
+  int32_t owner_block;
+  int32_t neighbor_block; /* -1: boundary */
+  int32_t sub;
+  int64_t count;
+  const int32_t* owner_elem;
+  const int32_t* neighbor_elem;
+  const void* geom; /* (F, 4): unit normal owner->neighbour, area */
+  const int32_t* owner_table;
+  const int32_t* neighbor_table;
+  const void* owner_tables;
+  const void* neighbor_tables;
+  const int32_t* owner_sup;
+  const int32_t* neighbor_sup;
+  const void* face_weights; /* (Qf) normalized face quadrature weights */
+  const int32_t* bflags;    /* (F) FSG_BND_* flags (boundary groups) */
+  int64_t out_owner;        /* offset (reals) of the owner records in face_out */
+  int64_t out_neighbor;
+} fsg_group_desc;
+
+/* Boundary object (FarfieldWallBoundary, hybrid/boundary.py:6-102).
+ * has_boundary = 0 reproduces boundary_state=None (U_ghost = U_inside). */
+typedef struct {
+  int32_t has_boundary;
+  int32_t has_wall_temperature;
+  double wall_temperature;
+  double farfield[16]; /* primitive [rho, u, v, w, p, Y...] */
+} fsg_boundary;
+
+/* Admissibility diagnostics written by a stage (device memory), as
+ * returned by hybrid/time_integration.py:291-312: valid == (nonfinite == 0
+ * && rho_min >= floor && p_min >= floor). */
+typedef struct {
+  double rho_min;
+  double p_min;
+  uint32_t nonfinite;
+  uint32_t reserved;
+} fsg_stage_diag;
+
+/* Builds a context over caller-owned tables.  face_out is a device scratch
+ * buffer of the context dtype sized by the caller (sum over groups of
+ * count * (NF_owner + NF_neighbor) * (nvars + 12) reals).
+ * Replaces the set-up of HybridEulerOperator / HybridNavierStokesOperator
+ * (hybrid/spatial.py:11-18, hybrid/navier_stokes.py:10-25). */
+int fsg_create(const fsg_params* params, int32_t nblocks, const fsg_block_desc* blocks,
+               int32_t ngroups, const fsg_group_desc* groups, const int32_t* support,
+               void* face_out, fsg_context** out);
+
+/* Residual or rhs of the listed terms for every block.
+ * mode FSG_MODE_RESIDUAL, terms EULER     = HybridEulerOperator.residual (hybrid/spatial.py:137)
+ * mode FSG_MODE_RHS,      terms EULER     = inviscid_rhs (hybrid/navier_stokes.py:345)
+ * mode FSG_MODE_RESIDUAL, terms VISCOUS   = viscous_residual (hybrid/navier_stokes.py:210)
+ * mode FSG_MODE_RHS,      terms VISCOUS   = viscous_rhs (hybrid/navier_stokes.py:326)
+ * mode FSG_MODE_RHS,      terms both      = rhs (hybrid/navier_stokes.py:338)
+ * U[b] and out[b] are (E_b, N, nvars) arrays of block b. */
+int fsg_residual(fsg_context* ctx, const void* const* U, void* const* out, int32_t mode,
+                 int32_t terms, const fsg_boundary* boundary, void* stream);
+
+/* One SSP-RK stage fused into the residual epilogue
+ * (hybrid/time_integration.py:404-439 without a limiter):
+ *   next = alpha * base + beta * (curr + dt * rhs(curr)),
+ * followed by the node admissibility reductions into diag (device).  dt is
+ * a scalar unless dt_elem is non-null (one dt per element, the LTS dict).
+ * next may alias curr or base. */
+int fsg_rk_stage(fsg_context* ctx, const void* const* base, const void* const* curr,
+                 void* const* next, double dt, const void* const* dt_elem, double alpha,
+                 double beta, int32_t terms, const fsg_boundary* boundary,
+                 fsg_stage_diag* diag, void* stream);
+
+/* Sets diag to (+inf, +inf, 0) on the stream (a fill kernel, capture-safe). */
+int fsg_diag_reset(fsg_stage_diag* diag, void* stream);
+
+/* Per-launch timing with HIP events recorded on the launch stream around each
+ * face-group and element-block kernel (no counterpart in the reference; used
+ * by the benchmark to report a kernel's own duration).  fsg_timing(ctx, 1)
+ * starts a fresh recording; fsg_timing_read waits for the recorded events,
+ * writes the summed milliseconds and launch counts per group / block (arrays
+ * of ngroups / nblocks entries, any may be null) and clears the record. */
+int fsg_timing(fsg_context* ctx, int32_t enable);
+int fsg_timing_read(fsg_context* ctx, double* group_ms, double* block_ms,
+                    int64_t* group_launches, int64_t* block_launches);
+
+void fsg_destroy(fsg_context* ctx);
+const char* fsg_last_error(void);
+int32_t fsg_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+// Launch dispatch from runtime (element type, face type, degree, nvars,
+// precision) to the compile-time kernel instantiations.
+#pragma once
+#include "fsg_kernels.h"
+
+namespace fsg {
+
+constexpr int ELEM_THREADS = 128;
+
+template <int P, int V, typename real>
+int launch_element(int type, int64_t count, const ElemArgs<real>& a, const Phys<real>& ph,
+                   hipStream_t s);
+
+template <int P, int V, typename real>
+int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64_t count,
+                const FaceArgs<real>& a, const Phys<real>& ph, hipStream_t s);
+
+#define FSG_DEFINE_LAUNCHERS(P, V, real)                                                        \
+  template <>                                                                                   \
+  int launch_element<P, V, real>(int type, int64_t count, const ElemArgs<real>& a,              \
+                                 const Phys<real>& ph, hipStream_t s) {                        \
+    if (count <= 0) return 0;                                                                   \
+    dim3 grid((unsigned)count), block(ELEM_THREADS);                                            \
+    switch (type) {                                                                             \
+      case HEX: hipLaunchKernelGGL((element_kernel<HEX, P, V, real, ELEM_THREADS>), grid, block, 0, s, a, ph); break; \
+      case WEDGE: hipLaunchKernelGGL((element_kernel<WEDGE, P, V, real, ELEM_THREADS>), grid, block, 0, s, a, ph); break; \
+      case TET: hipLaunchKernelGGL((element_kernel<TET, P, V, real, ELEM_THREADS>), grid, block, 0, s, a, ph); break; \
+      default: return -2;                                                                       \
+    }                                                                                           \
+    return hipGetLastError() == hipSuccess ? 0 : -1;                                            \
+  }                                                                                             \
+  template <>                                                                                   \
+  int launch_face<P, V, real>(int ft, int to, int tn, int sub, int64_t count,                   \
+                              const FaceArgs<real>& a, const Phys<real>& ph, hipStream_t s) {   \
+    if (count <= 0) return 0;                                                                   \
+    dim3 grid((unsigned)count), block(64);                                                      \
+    const int key = ((ft * 2 + sub) * 4 + to) * 4 + (tn + 1);                                   \
+    switch (key) {                                                                              \
+      FSG_FACE_CASE(QUAD, 0, HEX, HEX, P, V, real)                                              \
+      FSG_FACE_CASE(QUAD, 0, HEX, WEDGE, P, V, real)                                            \
+      FSG_FACE_CASE(QUAD, 0, HEX, -1, P, V, real)                                               \
+      FSG_FACE_CASE(QUAD, 0, WEDGE, HEX, P, V, real)                                            \
+      FSG_FACE_CASE(QUAD, 0, WEDGE, WEDGE, P, V, real)                                          \
+      FSG_FACE_CASE(QUAD, 0, WEDGE, -1, P, V, real)                                             \
+      FSG_FACE_CASE(TRI, 0, WEDGE, WEDGE, P, V, real)                                           \
+      FSG_FACE_CASE(TRI, 0, WEDGE, TET, P, V, real)                                             \
+      FSG_FACE_CASE(TRI, 0, WEDGE, -1, P, V, real)                                              \
+      FSG_FACE_CASE(TRI, 0, TET, WEDGE, P, V, real)                                             \
+      FSG_FACE_CASE(TRI, 0, TET, TET, P, V, real)                                               \
+      FSG_FACE_CASE(TRI, 0, TET, -1, P, V, real)                                                \
+      FSG_FACE_CASE(TRI, 1, HEX, WEDGE, P, V, real)                                             \
+      FSG_FACE_CASE(TRI, 1, HEX, TET, P, V, real)                                               \
+      FSG_FACE_CASE(TRI, 1, WEDGE, WEDGE, P, V, real)                                           \
+      FSG_FACE_CASE(TRI, 1, WEDGE, TET, P, V, real)                                             \
+      default: return -3;                                                                       \
+    }                                                                                           \
+    return hipGetLastError() == hipSuccess ? 0 : -1;                                            \
+  }
+
+#define FSG_FACE_CASE(FT, SUB, TO, TN, P, V, real)                                             \
+  case ((FT * 2 + SUB) * 4 + TO) * 4 + (TN + 1):                                               \
+    hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real>), grid, block, 0, s, a, ph); \
+    break;
+
+}  // namespace fsg

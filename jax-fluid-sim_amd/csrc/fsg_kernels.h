@@ -1,0 +1,590 @@
+// Hybrid DG residual kernels for gfx950.
+//
+// One RK stage (or one residual/rhs evaluation) is two launches per element
+// type / face group:
+//
+//  1. face_kernel: one wavefront per face.  It gathers both cells' nodal
+//     states into LDS, evaluates anchored Euler traces and HLLC
+//     (hybrid/spatial.py:142-232), the BR2 jump, lift right-hand sides and
+//     face gradients (hybrid/navier_stokes.py:138-188, 236-323), and writes
+//     each side's surface contribution and lift right-hand side on that
+//     side's face-support nodes only (the basis of a node off the face is
+//     zero there).
+//  2. element_kernel: one workgroup per element.  It gathers its face-side
+//     records (a fixed CSR, no atomics), forms the BR2 lift, evaluates the
+//     Euler and viscous volume integrals (hybrid/spatial.py:47-72,
+//     hybrid/navier_stokes.py:215-234), applies the sliver-scaled inverse
+//     mass matrix (hybrid/geometry.py:181-192) and, in stage mode, the SSP-RK
+//     combination and admissibility reductions of
+//     hybrid/time_integration.py:264-312, 404-439.
+#pragma once
+#include "fsg_physics.h"
+
+namespace fsg {
+
+struct StageDiag {         // device-side admissibility diagnostics
+  double rho_min;
+  double p_min;
+  unsigned int nonfinite;
+  unsigned int pad;
+};
+
+template <typename real>
+struct ElemArgs {
+  const real* U;          // (E, N, V) current state
+  const real* Ubase;      // (E, N, V) stage base state (stage mode)
+  real* out;              // (E, N, V) residual / rhs / next state
+  const real* Bt;         // (N, Q)   basis at volume points, transposed
+  const real* Dt;         // (N, 3, Q) reference gradients, q fastest
+  const real* D;          // (Q, 3, N) reference gradients, node fastest
+  const real* wq;         // (Q)
+  const real* minv;       // (N, N) reference inverse mass
+  const real* jinv;       // (E, 9)  [r*3+d] = d xi_r / d x_d
+  const real* scale;      // (E) min(1, V/1e-4) / |det J|
+  const real* detw;       // (E) |det J|
+  const int64_t* rec_ptr; // (E+1)
+  const int64_t* rec_off; // face-output offsets (reals)
+  const int32_t* rec_sup; // offsets into the support-index array
+  const int32_t* rec_nf;
+  const int32_t* support;
+  const real* face_out;
+  const real* dt_elem;    // (E) per-element dt or nullptr
+  real dt, alpha, beta;
+  int mode;               // 0 residual, 1 rhs, 2 stage
+  int terms;
+  StageDiag* diag;
+};
+
+template <typename real>
+struct FaceArgs {
+  const real* Uo;          // owner block state
+  const real* Un;          // neighbour block state
+  const int32_t* owner_elem;
+  const int32_t* neighbor_elem;
+  const real* geom;        // (F, 4) normal, area
+  const int32_t* owner_table;
+  const int32_t* neighbor_table;
+  const real* owner_tables;
+  const real* neighbor_tables;
+  const int32_t* owner_sup;     // (tables, FO)
+  const int32_t* neighbor_sup;  // (tables, FN)
+  const real* fw;          // (Qf) normalised face weights x reference measure
+  const int32_t* bflags;   // (F) boundary flags
+  const real* jinv_o;
+  const real* scale_o;
+  const real* jinv_n;
+  const real* scale_n;
+  real* out_o;             // (F, FO, V+12)
+  real* out_n;             // (F, FN, V+12)
+  int64_t count;
+  int terms;
+};
+
+template <typename real>
+__device__ __forceinline__ void atomic_min_nan(double* addr, double v) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(addr);
+  unsigned long long old = *p, assumed;
+  do {
+    assumed = old;
+    double cur = __longlong_as_double(static_cast<long long>(assumed));
+    bool take = (v != v) ? (cur == cur) : (v < cur);
+    if (!take) return;
+    old = atomicCAS(p, assumed, static_cast<unsigned long long>(__double_as_longlong(v)));
+  } while (old != assumed);
+}
+
+__device__ __forceinline__ double nan_min(double a, double b) {
+  if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000LL);
+  return a < b ? a : b;
+}
+
+template <int T, int P, int V, typename real, int NT>
+__global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real> ph) {
+  constexpr int N = n_nodes(T, P);
+  constexpr int Q = n_vol_q(T, P);
+  constexpr int W = V + 12;
+  __shared__ real sU[N * V];
+  __shared__ real sR[N * V];
+  __shared__ real sv[N * 4];
+  __shared__ real sb[N * 12];
+  __shared__ real sL[N * 12];
+  __shared__ real sF[Q * 3 * V];
+  __shared__ double red[2][NT / 64];
+  __shared__ unsigned int redn[NT / 64];
+
+  const int tid = threadIdx.x;
+  const int64_t e = blockIdx.x;
+  const bool euler = a.terms & TERM_EULER;
+  const bool visc = a.terms & TERM_VISCOUS;
+  const real gamma = ph.gamma;
+
+  const real* Ue = a.U + e * (N * V);
+  for (int k = tid; k < N * V; k += NT) {
+    sU[k] = Ue[k];
+    sR[k] = real(0);
+  }
+  for (int k = tid; k < N * 12; k += NT) sb[k] = real(0);
+  __syncthreads();
+  if (visc)
+    for (int i = tid; i < N; i += NT)
+      viscous_variables<V, real>(sU + i * V, gamma, ph.gas_constant, sv + i * 4);
+
+  // Face-side records: surface terms into sR, lift right-hand sides into sb.
+  const int64_t r0 = a.rec_ptr[e], r1 = a.rec_ptr[e + 1];
+  for (int64_t r = r0; r < r1; ++r) {
+    const real* rec = a.face_out + a.rec_off[r];
+    const int32_t* sup = a.support + a.rec_sup[r];
+    const int nf = a.rec_nf[r];
+    const int width = visc ? W : V;
+    for (int k = tid; k < nf * width; k += NT) {
+      int kk = k / width, c = k - kk * width;
+      int node = sup[kk];
+      real val = rec[kk * W + c];
+      if (c < V)
+        sR[node * V + c] += val;
+      else
+        sb[node * 12 + (c - V)] += val;
+    }
+    __syncthreads();
+  }
+
+  const real s = a.scale[e];
+  const real dw = a.detw[e];
+  real J[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = a.jinv[e * 9 + k];
+
+  // BR2 global lift r = M_phys^-1 b  (hybrid/navier_stokes.py:157-186).
+  if (visc) {
+    for (int k = tid; k < N * 12; k += NT) {
+      int i = k / 12, c = k - i * 12;
+      real acc = real(0);
+      for (int j = 0; j < N; ++j) acc += a.minv[i * N + j] * sb[j * 12 + c];
+      sL[k] = s * acc;
+    }
+  }
+  __syncthreads();
+
+  // Volume quadrature points.
+  for (int q = tid; q < Q; q += NT) {
+    real dF[3 * V];
+#pragma unroll
+    for (int k = 0; k < 3 * V; ++k) dF[k] = real(0);
+    real Ua[V], Up[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      Ua[v] = real(0);
+      Up[v] = real(0);
+    }
+    for (int i = 0; i < N; ++i) {
+      real b = a.Bt[i * Q + q];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        real u = sU[i * V + v];
+        Up[v] += b * u;
+        Ua[v] += b * (u - sU[v]);
+      }
+    }
+    if (euler) {
+      // Anchored interpolation and baseline flux (hybrid/spatial.py:41-70).
+#pragma unroll
+      for (int v = 0; v < V; ++v) Ua[v] += sU[v];
+      Prim<V, real> wq = c2p<V, real>(Ua, gamma);
+      Prim<V, real> wa = c2p<V, real>(sU, gamma);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        real Fq[V], Fa[V];
+        axis_flux<V, real>(Ua, wq, d, Fq);
+        axis_flux<V, real>(sU, wa, d, Fa);
+#pragma unroll
+        for (int v = 0; v < V; ++v) dF[d * V + v] = Fq[v] - Fa[v];
+      }
+    }
+    if (visc) {
+      real G[12], g[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        G[k] = real(0);
+        g[k] = real(0);
+      }
+      for (int i = 0; i < N; ++i) {
+        real dx = a.Dt[(i * 3 + 0) * Q + q];
+        real dy = a.Dt[(i * 3 + 1) * Q + q];
+        real dz = a.Dt[(i * 3 + 2) * Q + q];
+        real b = a.Bt[i * Q + q];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          real vm = sv[i * 4 + m];
+          G[m * 3 + 0] += dx * vm;
+          G[m * 3 + 1] += dy * vm;
+          G[m * 3 + 2] += dz * vm;
+        }
+#pragma unroll
+        for (int c = 0; c < 12; ++c) g[c] += b * sL[i * 12 + c];
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          g[m * 3 + d] += G[m * 3 + 0] * J[0 * 3 + d] + G[m * 3 + 1] * J[1 * 3 + d] +
+                          G[m * 3 + 2] * J[2 * 3 + d];
+      real Fv[3 * V];
+      viscous_flux<V, real>(Up, g, gamma, ph.mu, ph.kappa, Fv);
+#pragma unroll
+      for (int k = 0; k < 3 * V; ++k) dF[k] -= Fv[k];
+    }
+    // Pull back to reference directions with the quadrature weight.
+    const real w = a.wq[q] * dw;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        sF[(q * 3 + r) * V + v] =
+            w * (J[r * 3 + 0] * dF[0 * V + v] + J[r * 3 + 1] * dF[1 * V + v] +
+                 J[r * 3 + 2] * dF[2 * V + v]);
+  }
+  __syncthreads();
+
+  // R_i += sum_q sum_r dphi_i/dxi_r F^_r
+  for (int k = tid; k < N * V; k += NT) {
+    int i = k / V, v = k - i * V;
+    real acc = real(0);
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) acc += a.D[(q * 3 + r) * N + i] * sF[(q * 3 + r) * V + v];
+    sR[k] += acc;
+  }
+  __syncthreads();
+
+  real* out = a.out + e * (N * V);
+  if (a.mode == 0) {
+    for (int k = tid; k < N * V; k += NT) out[k] = sR[k];
+    return;
+  }
+  const real dte = a.dt_elem ? a.dt_elem[e] : a.dt;
+  for (int k = tid; k < N * V; k += NT) {
+    int i = k / V, v = k - i * V;
+    real acc = real(0);
+    for (int j = 0; j < N; ++j) acc += a.minv[i * N + j] * sR[j * V + v];
+    real rhs = s * acc;
+    if (a.mode == 1) {
+      out[k] = rhs;
+    } else {
+      real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (sU[k] + dte * rhs);
+      out[k] = nxt;
+      sF[k] = nxt;
+    }
+  }
+  if (a.mode != 2) return;
+  __syncthreads();
+
+  // Admissibility at nodes (hybrid/time_integration.py:291-312).
+  double rmn = __longlong_as_double(0x7ff0000000000000LL), pmn = rmn;
+  unsigned int bad = 0;
+  for (int i = tid; i < N; i += NT) {
+    const real* u = sF + i * V;
+    for (int v = 0; v < V; ++v) bad += finite(u[v]) ? 0u : 1u;
+    real rho = u[0];
+    real ke = real(0.5) * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / rmax(rho, real(1.0e-14));
+    real p = (gamma - real(1)) * (u[4] - ke);
+    rmn = nan_min(rmn, double(rho));
+    pmn = nan_min(pmn, double(p));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    rmn = nan_min(rmn, __shfl_down(rmn, off, 64));
+    pmn = nan_min(pmn, __shfl_down(pmn, off, 64));
+    bad += __shfl_down(bad, off, 64);
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  if (lane == 0) {
+    red[0][wv] = rmn;
+    red[1][wv] = pmn;
+    redn[wv] = bad;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int k = 1; k < NT / 64; ++k) {
+      rmn = nan_min(rmn, red[0][k]);
+      pmn = nan_min(pmn, red[1][k]);
+      bad += redn[k];
+    }
+    atomic_min_nan<real>(&a.diag->rho_min, rmn);
+    atomic_min_nan<real>(&a.diag->p_min, pmn);
+    if (bad) atomicAdd(&a.diag->nonfinite, bad);
+  }
+}
+
+template <int FT, int TO, int TN, int SUB, int P, int V, typename real>
+__global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> ph) {
+  constexpr int NO = n_nodes(TO, P);
+  constexpr int NN = TN >= 0 ? n_nodes(TN, P) : 1;
+  constexpr int QF = n_face_q(FT, P);
+  constexpr int FO = n_face_nodes(FT, SUB, P);
+  constexpr int FN = TN >= 0 ? n_face_nodes(FT, 0, P) : 1;
+  constexpr int TSO = QF * FO * 2 + QF * NO * 3;
+  constexpr int TSN = QF * FN * 2 + QF * NN * 3;
+  constexpr int W = V + 12;
+  constexpr bool NBR = TN >= 0;
+
+  __shared__ real sUo[NO * V], sUn[NN * V], svo[NO * 4], svn[NN * 4];
+  __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
+  __shared__ real sbo[FO * 12], sbn[FN * 12], sgo[QF * 12], sgn[QF * 12];
+
+  const int tid = threadIdx.x;
+  const int64_t f = blockIdx.x;
+  const bool euler = a.terms & TERM_EULER;
+  const bool visc = a.terms & TERM_VISCOUS;
+  const real gamma = ph.gamma;
+
+  const int64_t eo = a.owner_elem[f];
+  const int64_t en = NBR ? a.neighbor_elem[f] : 0;
+  const real n[3] = {a.geom[f * 4 + 0], a.geom[f * 4 + 1], a.geom[f * 4 + 2]};
+  const real area = a.geom[f * 4 + 3];
+  const int flags = NBR ? 0 : a.bflags[f];
+  const int to_id = a.owner_table[f];
+  const real* tbo = a.owner_tables + (int64_t)to_id * TSO;
+  const int32_t* supo = a.owner_sup + to_id * FO;
+  const real* tbn = nullptr;
+  const int32_t* supn = nullptr;
+  if constexpr (NBR) {
+    const int tn_id = a.neighbor_table[f];
+    tbn = a.neighbor_tables + (int64_t)tn_id * TSN;
+    supn = a.neighbor_sup + tn_id * FN;
+  }
+
+  for (int k = tid; k < NO * V; k += 64) sUo[k] = a.Uo[eo * (NO * V) + k];
+  if constexpr (NBR)
+    for (int k = tid; k < NN * V; k += 64) sUn[k] = a.Un[en * (NN * V) + k];
+  __syncthreads();
+  if (visc) {
+    for (int i = tid; i < NO; i += 64) viscous_variables<V, real>(sUo + i * V, gamma, ph.gas_constant, svo + i * 4);
+    if constexpr (NBR)
+      for (int i = tid; i < NN; i += 64) viscous_variables<V, real>(sUn + i * V, gamma, ph.gas_constant, svn + i * 4);
+  }
+  __syncthreads();
+
+  // Phase 1: traces and the Euler numerical flux at each face point.
+  for (int q = tid; q < QF; q += 64) {
+    const real w = a.fw[q] * area;
+    const real* phio = tbo + q * FO;
+    real Ua_o[V], Up_o[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      Ua_o[v] = real(0);
+      Up_o[v] = real(0);
+    }
+#pragma unroll
+    for (int k = 0; k < FO; ++k) {
+      real b = phio[k];
+      const real* u = sUo + supo[k] * V;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        Up_o[v] += b * u[v];
+        Ua_o[v] += b * (u[v] - sUo[v]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) Ua_o[v] += sUo[v];
+    real Ua_n[V], Up_n[V];
+    if constexpr (NBR) {
+      const real* phin = tbn + q * FN;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        Ua_n[v] = real(0);
+        Up_n[v] = real(0);
+      }
+#pragma unroll
+      for (int k = 0; k < FN; ++k) {
+        real b = phin[k];
+        const real* u = sUn + supn[k] * V;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          Up_n[v] += b * u[v];
+          Ua_n[v] += b * (u[v] - sUn[v]);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) Ua_n[v] += sUn[v];
+    } else {
+      ghost_state<V, real>(Ua_o, n, flags, ph, Ua_n);
+#pragma unroll
+      for (int v = 0; v < V; ++v) Up_n[v] = Up_o[v];  // unused: boundary flux is the owner's
+    }
+    real So[V], Sn[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      So[v] = real(0);
+      Sn[v] = real(0);
+    }
+    if (euler) {
+      real Fs[V], Bo[V];
+      if (ph.rusanov) rusanov<V, real>(Ua_o, Ua_n, n, gamma, Fs);
+      else hllc<V, real>(Ua_o, Ua_n, n, gamma, Fs);
+      Prim<V, real> wa = c2p<V, real>(sUo, gamma);
+      normal_flux<V, real>(sUo, wa, n, Bo);
+#pragma unroll
+      for (int v = 0; v < V; ++v) So[v] = -w * (Fs[v] - Bo[v]);
+      if constexpr (NBR) {
+        real Bn[V];
+        Prim<V, real> wb = c2p<V, real>(sUn, gamma);
+        normal_flux<V, real>(sUn, wb, n, Bn);
+#pragma unroll
+        for (int v = 0; v < V; ++v) Sn[v] = w * (Fs[v] - Bn[v]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      sSo[q * V + v] = So[v];
+      sSn[q * V + v] = Sn[v];
+      sPo[q * V + v] = Up_o[v];
+      sPn[q * V + v] = Up_n[v];
+    }
+    if (visc) {
+      real vo[4] = {0, 0, 0, 0}, vn[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < FO; ++k) {
+        real b = phio[k];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) vo[m] += b * svo[supo[k] * 4 + m];
+      }
+      if constexpr (NBR) {
+        const real* phin = tbn + q * FN;
+#pragma unroll
+        for (int k = 0; k < FN; ++k) {
+          real b = phin[k];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) vn[m] += b * svn[supn[k] * 4 + m];
+        }
+      } else {
+        ghost_viscous<V, real>(Up_o, n, flags, ph, vn);
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) sJ[q * 4 + m] = w * (vn[m] - vo[m]);
+    }
+  }
+  __syncthreads();
+
+  if (visc) {
+    // Phase 2: lift right-hand sides b = 1/2 sum_q w [[v]] (x) n phi.
+    for (int k = tid; k < FO * 4; k += 64) {
+      int kk = k >> 2, m = k & 3;
+      real acc = real(0);
+      for (int q = 0; q < QF; ++q) acc += tbo[q * FO + kk] * sJ[q * 4 + m];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sbo[kk * 12 + m * 3 + d] = real(0.5) * acc * n[d];
+    }
+    if constexpr (NBR) {
+      for (int k = tid; k < FN * 4; k += 64) {
+        int kk = k >> 2, m = k & 3;
+        real acc = real(0);
+        for (int q = 0; q < QF; ++q) acc += tbn[q * FN + kk] * sJ[q * 4 + m];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sbn[kk * 12 + m * 3 + d] = real(0.5) * acc * n[d];
+      }
+    }
+    __syncthreads();
+
+    // Phase 3: face gradients = raw polynomial gradient + eta * local lift.
+    const real so = a.scale_o[eo];
+    real Jo[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
+    for (int k = tid; k < QF * 4; k += 64) {
+      int q = k >> 2, m = k & 3;
+      const real* dph = tbo + QF * FO * 2 + q * NO * 3;
+      real G0 = 0, G1 = 0, G2 = 0;
+      for (int i = 0; i < NO; ++i) {
+        real vm = svo[i * 4 + m];
+        G0 += dph[i * 3 + 0] * vm;
+        G1 += dph[i * 3 + 1] * vm;
+        G2 += dph[i * 3 + 2] * vm;
+      }
+      const real* lop = tbo + QF * FO + q * FO;
+      real L[3] = {0, 0, 0};
+      for (int kk = 0; kk < FO; ++kk) {
+        real c = lop[kk];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) L[d] += c * sbo[kk * 12 + m * 3 + d];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        sgo[q * 12 + m * 3 + d] = G0 * Jo[0 * 3 + d] + G1 * Jo[1 * 3 + d] + G2 * Jo[2 * 3 + d] +
+                                  ph.eta * (so * L[d]);
+    }
+    if constexpr (NBR) {
+      const real sn = a.scale_n[en];
+      real Jn[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
+      for (int k = tid; k < QF * 4; k += 64) {
+        int q = k >> 2, m = k & 3;
+        const real* dph = tbn + QF * FN * 2 + q * NN * 3;
+        real G0 = 0, G1 = 0, G2 = 0;
+        for (int i = 0; i < NN; ++i) {
+          real vm = svn[i * 4 + m];
+          G0 += dph[i * 3 + 0] * vm;
+          G1 += dph[i * 3 + 1] * vm;
+          G2 += dph[i * 3 + 2] * vm;
+        }
+        const real* lop = tbn + QF * FN + q * FN;
+        real L[3] = {0, 0, 0};
+        for (int kk = 0; kk < FN; ++kk) {
+          real c = lop[kk];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) L[d] += c * sbn[kk * 12 + m * 3 + d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          sgn[q * 12 + m * 3 + d] = G0 * Jn[0 * 3 + d] + G1 * Jn[1 * 3 + d] +
+                                    G2 * Jn[2 * 3 + d] + ph.eta * (sn * L[d]);
+      }
+    }
+    __syncthreads();
+
+    // Phase 4: central viscous flux (owner flux only on the boundary).
+    for (int q = tid; q < QF; q += 64) {
+      const real w = a.fw[q] * area;
+      real Fo[3 * V], fn[V];
+      viscous_flux<V, real>(sPo + q * V, sgo + q * 12, gamma, ph.mu, ph.kappa, Fo);
+      if constexpr (NBR) {
+        real Fn[3 * V];
+        viscous_flux<V, real>(sPn + q * V, sgn + q * 12, gamma, ph.mu, ph.kappa, Fn);
+#pragma unroll
+        for (int k = 0; k < 3 * V; ++k) Fo[k] = real(0.5) * (Fo[k] + Fn[k]);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) fn[v] = Fo[v] * n[0] + Fo[V + v] * n[1] + Fo[2 * V + v] * n[2];
+      if (!NBR && ph.has_boundary && (flags & BND_WALL)) fn[4] = real(0);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        sSo[q * V + v] += w * fn[v];
+        sSn[q * V + v] -= w * fn[v];
+      }
+    }
+    __syncthreads();
+  }
+
+  // Phase 5: project onto the face-support basis of each side.
+  real* oo = a.out_o + f * (FO * W);
+  for (int k = tid; k < FO * V; k += 64) {
+    int kk = k / V, v = k - kk * V;
+    real acc = real(0);
+    for (int q = 0; q < QF; ++q) acc += tbo[q * FO + kk] * sSo[q * V + v];
+    oo[kk * W + v] = acc;
+  }
+  if (visc)
+    for (int k = tid; k < FO * 12; k += 64) oo[(k / 12) * W + V + (k % 12)] = sbo[k];
+  if constexpr (NBR) {
+    real* on = a.out_n + f * (FN * W);
+    for (int k = tid; k < FN * V; k += 64) {
+      int kk = k / V, v = k - kk * V;
+      real acc = real(0);
+      for (int q = 0; q < QF; ++q) acc += tbn[q * FN + kk] * sSn[q * V + v];
+      on[kk * W + v] = acc;
+    }
+    if (visc)
+      for (int k = tid; k < FN * 12; k += 64) on[(k / 12) * W + V + (k % 12)] = sbn[k];
+  }
+}
+
+}  // namespace fsg

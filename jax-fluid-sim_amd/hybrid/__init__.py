@@ -1,1 +1,17 @@
-"""MI355X-native hybrid DG hot path (host mirror of the reference hybrid package)."""
+"""MI355X-native hybrid DG hot path.
+
+Host mirror of the reference's ``hybrid`` package for the residual / RK-stage
+path: the classes keep the reference names and signatures and call the gfx950
+kernels of libfluid_gfx950.so.
+"""
+
+from .boundary import FarfieldWallBoundary
+from .meshgen import box_mesh, smoke_mesh
+from .solver import CompressibleStateLayout, HybridCompressibleSolver
+from .spatial import HybridEulerOperator, HybridNavierStokesOperator
+from .time_integration import HybridSSPRK3
+from .topology import Topology
+
+__all__ = ["FarfieldWallBoundary", "box_mesh", "smoke_mesh", "CompressibleStateLayout",
+           "HybridCompressibleSolver", "HybridEulerOperator", "HybridNavierStokesOperator",
+           "HybridSSPRK3", "Topology"]

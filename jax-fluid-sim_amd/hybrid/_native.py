@@ -1,0 +1,103 @@
+"""ctypes binding of libfluid_gfx950.so (C ABI in include/fluid_gfx950.h).
+
+The library is loaded after torch so both share torch's HIP runtime.  There
+is no CPU fallback: a missing or unloadable library raises.
+"""
+
+import ctypes
+from pathlib import Path
+
+import torch  # noqa: F401  (loads the HIP runtime the library binds to)
+
+LIB_PATH = Path(__file__).resolve().parent / "libfluid_gfx950.so"
+
+F64, F32 = 0, 1
+MODE_RESIDUAL, MODE_RHS = 0, 1
+TERM_EULER, TERM_VISCOUS = 1, 2
+BND_WALL, BND_INLET, BND_NOSLIP, BND_SLIP = 1, 2, 4, 8
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("gas_constant", ctypes.c_double),
+                ("dynamic_viscosity", ctypes.c_double),
+                ("thermal_conductivity", ctypes.c_double), ("br2_penalty", ctypes.c_double),
+                ("inviscid_flux", ctypes.c_int32), ("nvars", ctypes.c_int32),
+                ("degree", ctypes.c_int32), ("dtype", ctypes.c_int32)]
+
+
+class BlockDesc(ctypes.Structure):
+    _fields_ = [("element_type", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("count", ctypes.c_int64)] + [
+        (k, ctypes.c_void_p) for k in ("Bt", "Dt", "D", "wq", "minv", "jinv", "scale", "detw",
+                                       "rec_ptr", "rec_off", "rec_sup", "rec_nf")]
+
+
+class GroupDesc(ctypes.Structure):
+    _fields_ = [("face_type", ctypes.c_int32), ("owner_block", ctypes.c_int32),
+                ("neighbor_block", ctypes.c_int32), ("sub", ctypes.c_int32),
+                ("count", ctypes.c_int64)] + [
+        (k, ctypes.c_void_p) for k in ("owner_elem", "neighbor_elem", "geom", "owner_table",
+                                       "neighbor_table", "owner_tables", "neighbor_tables",
+                                       "owner_sup", "neighbor_sup", "face_weights", "bflags")] + [
+        ("out_owner", ctypes.c_int64), ("out_neighbor", ctypes.c_int64)]
+
+
+class Boundary(ctypes.Structure):
+    _fields_ = [("has_boundary", ctypes.c_int32), ("has_wall_temperature", ctypes.c_int32),
+                ("wall_temperature", ctypes.c_double), ("farfield", ctypes.c_double * 16)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with jax-fluid-sim_amd/build.py")
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        L.fsg_create.argtypes = [ctypes.POINTER(Params), i32, ctypes.POINTER(BlockDesc), i32,
+                                 ctypes.POINTER(GroupDesc), vp, vp, ctypes.POINTER(vp)]
+        L.fsg_create.restype = ctypes.c_int
+        L.fsg_residual.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, i32,
+                                   ctypes.POINTER(Boundary), vp]
+        L.fsg_residual.restype = ctypes.c_int
+        L.fsg_rk_stage.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                   f64, ctypes.POINTER(vp), f64, f64, i32,
+                                   ctypes.POINTER(Boundary), vp, vp]
+        L.fsg_rk_stage.restype = ctypes.c_int
+        L.fsg_diag_reset.argtypes = [vp, vp]
+        L.fsg_diag_reset.restype = ctypes.c_int
+        L.fsg_timing.argtypes = [vp, i32]
+        L.fsg_timing.restype = ctypes.c_int
+        L.fsg_timing_read.argtypes = [vp, vp, vp, vp, vp]
+        L.fsg_timing_read.restype = ctypes.c_int
+        L.fsg_destroy.argtypes = [vp]
+        L.fsg_destroy.restype = None
+        L.fsg_last_error.argtypes = []
+        L.fsg_last_error.restype = ctypes.c_char_p
+        L.fsg_abi_version.restype = ctypes.c_int32
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(f"libfluid_gfx950 error {rc}: {lib().fsg_last_error().decode()}")
+
+
+def ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def ptr_array(tensors):
+    arr = (ctypes.c_void_p * len(tensors))()
+    for i, t in enumerate(tensors):
+        arr[i] = 0 if t is None else t.data_ptr()
+    return arr
+
+
+def stream_handle(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,158 @@
+"""Parity of the HIP residual / rhs / RK-stage path with the CPU FP64 oracle.
+
+FP64 bar: relative L2 error <= 1e-12 per conserved variable (the plain-path
+tolerance).  FP32 bar: <= 2e-5 per variable on the same inputs (single
+precision arithmetic throughout the kernels).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import PLANE_NAMES, perturbed_primitive, rel_l2, tgv_primitive
+
+from hybrid import FarfieldWallBoundary, HybridCompressibleSolver, HybridNavierStokesOperator, \
+    HybridSSPRK3, box_mesh, smoke_mesh
+from oracle.mesh import from_arrays
+from oracle.operators import NavierStokesOperator
+from oracle.solver import CompressibleSolver, FarfieldWallBoundary as OracleBoundary
+from oracle.time_integration import SSPRK3
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-5
+
+
+def _case(name, p, dtype=torch.float64):
+    mu = 0.0
+    nspecies = 0
+    boundary = False
+    if name == "tgv":
+        m = box_mesh(4, lengths=(2 * np.pi,) * 3, periodic=(True,) * 3)
+        mu = 1.0 / 1600.0
+        prim = tgv_primitive
+    elif name == "smoke":
+        m = smoke_mesh()
+        mu = 0.02
+        boundary = True
+        prim = lambda x: perturbed_primitive(x, seed=1)
+    elif name == "split":
+        m = box_mesh(4, split_fraction=0.35, seed=7, plane_names=PLANE_NAMES)
+        mu = 0.01
+        boundary = True
+        prim = lambda x: perturbed_primitive(x, seed=2)
+    elif name == "split_periodic":
+        m = box_mesh((4, 3, 4), lengths=(1.0, 2.0, 1.5), periodic=(True, True, True),
+                     split_fraction=0.4, seed=11)
+        mu = 0.005
+        prim = lambda x: perturbed_primitive(x, seed=3)
+    elif name == "species":
+        m = box_mesh(3, split_fraction=0.4, seed=5, plane_names=PLANE_NAMES)
+        nspecies = 11
+        mu = 0.01
+        boundary = True
+        prim = lambda x: perturbed_primitive(x, seed=4, nspecies=11)
+    else:
+        raise KeyError(name)
+    names = tuple(f"s{k}" for k in range(nspecies))
+    dev_solver = HybridCompressibleSolver(m, species_names=names, polynomial_degree=p,
+                                          dynamic_viscosity=mu, prandtl=0.71, dtype=dtype)
+    dev_op = HybridNavierStokesOperator(dev_solver)
+    om = from_arrays(m)
+    osolver = CompressibleSolver(gamma=1.4, dynamic_viscosity=mu, prandtl=0.71,
+                                 num_species=nspecies)
+    oop = NavierStokesOperator(om, osolver, p)
+    states = {t: osolver.primitive_to_conserved(prim(x))
+              for t, x in oop.geometry.solution_node_coordinates.items()}
+    dev_b = ob = None
+    if boundary:
+        far = np.concatenate([[1.2, 0.9, 0.1, -0.05, 1.1],
+                              np.full(nspecies, 1.0 / max(nspecies, 1))])
+        dev_b = FarfieldWallBoundary(dev_solver, far)
+        ob = OracleBoundary(osolver, far, inlet_tags=sorted(dev_b.inlet_tags),
+                            outlet_tags=sorted(dev_b.outlet_tags),
+                            wall_tags=sorted(dev_b.wall_tags),
+                            no_slip_tags=sorted(dev_b.no_slip_tags),
+                            slip_tags=sorted(dev_b.slip_tags))
+    return dev_solver, dev_op, dev_b, oop, ob, states
+
+
+def _np(d):
+    return {k: v.double().cpu().numpy() for k, v in d.items()}
+
+
+CASES = [("smoke", 1), ("smoke", 2), ("smoke", 3), ("tgv", 1), ("tgv", 2), ("tgv", 3),
+         ("split", 2), ("split", 3), ("split_periodic", 2), ("species", 2)]
+
+
+@pytest.mark.parametrize("name,p", CASES)
+def test_residual_and_rhs_fp64(name, p):
+    print(f"case {name} p={p}", flush=True)
+    ds, dop, db, oop, ob, states = _case(name, p)
+    U = ds.to_device(states)
+    got = _np(dop.residual(U, db))
+    ref = oop.residual(states, ob)
+    assert np.all(rel_l2(got, ref) <= FP64_TOL), rel_l2(got, ref)
+    got = _np(dop.viscous_residual(U, db))
+    ref = oop.viscous_residual(states, ob)
+    assert np.all(rel_l2(got, ref) <= FP64_TOL), rel_l2(got, ref)
+    got = _np(dop.rhs(U, db))
+    ref = oop.rhs(states, ob)
+    assert np.all(rel_l2(got, ref) <= FP64_TOL), rel_l2(got, ref)
+
+
+@pytest.mark.parametrize("name,p", [("tgv", 2), ("split", 2), ("smoke", 3)])
+def test_rk_stage_and_step_fp64(name, p):
+    print(f"case {name} p={p}", flush=True)
+    ds, dop, db, oop, ob, states = _case(name, p)
+    U = ds.to_device(states)
+    dt = 1e-3
+    oint = SSPRK3(oop)
+    dint = HybridSSPRK3(dop)
+    base = {k: v * 1.001 for k, v in states.items()}
+    ref, ok, rmin, pmin = oint.rk_stage(base, states, dt, 0.75, 0.25, ob)
+    got, gok, grmin, gpmin = dint.rk_stage(ds.to_device(base), U, dt, 0.75, 0.25, db)
+    assert np.all(rel_l2(_np(got), ref) <= FP64_TOL)
+    assert bool(gok) == ok
+    assert abs(float(grmin) - rmin) <= 1e-13 * abs(rmin)
+    assert abs(float(gpmin) - pmin) <= 1e-12 * abs(pmin)
+    ref, ok, rmin, pmin = oint.checked_step(states, dt, ob)
+    got, gok, grmin, gpmin = dint.checked_step(U, dt, db)
+    assert np.all(rel_l2(_np(got), ref) <= FP64_TOL)
+    assert bool(gok) == ok
+    assert abs(float(grmin) - rmin) <= 1e-12 * abs(rmin)
+
+
+@pytest.mark.parametrize("name,p", [("tgv", 3), ("split", 2), ("species", 2)])
+def test_rhs_fp32(name, p):
+    print(f"case {name} p={p} fp32", flush=True)
+    ds, dop, db, oop, ob, states = _case(name, p, dtype=torch.float32)
+    got = _np(dop.rhs(ds.to_device(states), db))
+    ref = oop.rhs(states, ob)
+    err = rel_l2(got, ref)
+    assert np.all(err <= FP32_TOL), err
+
+
+def test_free_stream_is_exact():
+    """A uniform state has an exactly zero Euler residual on a periodic mesh
+    (anchored traces + the identical-state HLLC shortcut)."""
+    m = box_mesh(3, periodic=(True,) * 3, split_fraction=0.3, seed=2)
+    ds = HybridCompressibleSolver(m, polynomial_degree=2)
+    op = HybridNavierStokesOperator(ds)
+    U = ds.uniform_state_blocks(1.3, (0.4, -0.2, 0.7), 2.1)
+    r = op.residual(U)
+    assert all(float(v.abs().max()) == 0.0 for v in r.values())
+
+
+def test_in_place_stage_matches_out_of_place():
+    ds, dop, db, oop, ob, states = _case("split", 2)
+    U = ds.to_device(states)
+    dint = HybridSSPRK3(dop)
+    a, _, _, _ = dint.rk_stage(U, U, 1e-3, 0.0, 1.0, db)
+    V = {k: v.clone() for k, v in U.items()}
+    import ctypes
+    from hybrid import _native as nat
+    dint._stage(V, V, V, 1e-3, 0.0, 1.0, db, True)
+    for k in a:
+        assert torch.equal(a[k], V[k])

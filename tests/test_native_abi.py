@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every function include/*.h declares;
+contexts are created over CPU-resident tables (no kernel runs without a GPU)."""
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(fs\w*)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    from hybrid import _native as nat
+    lib = nat.lib()
+    declared = _declared()
+    assert "fsg_create" in declared and "fsg_rk_stage" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.fsg_abi_version() >= 1
+
+
+def test_context_over_cpu_tables_and_argument_checks():
+    from conftest import PLANE_NAMES
+    from hybrid import HybridCompressibleSolver, HybridNavierStokesOperator, box_mesh
+    from hybrid import _native as nat
+    m = box_mesh(3, split_fraction=0.4, seed=3, plane_names=PLANE_NAMES)
+    s = HybridCompressibleSolver(m, polynomial_degree=2, dynamic_viscosity=0.01, device="cpu")
+    op = HybridNavierStokesOperator(s)
+    dm = op.device_mesh
+    assert dm.ctx.value
+    # every interior face contributes one record to each side, boundary faces one
+    f = s.mesh.faces
+    nrec = sum(int(b["rec_ptr"][-1]) for b in dm.block_tables)
+    assert nrec == 2 * int((f.neighbor_block >= 0).sum()) + int((f.neighbor_block < 0).sum())
+    # unsupported configuration is rejected with a message, not a crash
+    p = nat.Params()
+    p.degree, p.nvars, p.dtype = 4, 5, 0
+    import ctypes
+    ctx = ctypes.c_void_p()
+    rc = nat.lib().fsg_create(ctypes.byref(p), 0, None, 0, None, None, None, ctypes.byref(ctx))
+    assert rc != 0 and b"degree" in nat.lib().fsg_last_error()
+
+
+def test_support_tables_are_face_local():
+    """Each trace table's support nodes lie on the face and carry the whole
+    trace: the basis of every other node vanishes at the face points."""
+    from hybrid.device import _support
+    from hybrid.elements import LOCAL_FACES, basis, corners, face_quadrature
+    for t in ("hexahedron", "wedge", "tetra"):
+        for p in (1, 2, 3):
+            for lf, fv in enumerate(LOCAL_FACES[t]):
+                sup = _support(t, lf, p)
+                ft = "quad" if len(fv) == 4 else "triangle"
+                S, _ = face_quadrature(ft, p)
+                pts = S @ corners(t)[list(fv)]
+                phi = basis(t, p, pts)
+                off = np.setdiff1d(np.arange(phi.shape[1]), sup)
+                assert np.abs(phi[:, off]).max() < 1e-13
+                assert len(sup) == ((p + 1) ** 2 if ft == "quad" else (p + 1) * (p + 2) // 2)
