@@ -170,6 +170,39 @@ void fsg_destroy(fsg_context* ctx);
 const char* fsg_last_error(void);
 int32_t fsg_abi_version(void);
 
+/* ---- Park 11-species air chemistry (FP64) ---------------------------------
+ * table: device buffer of fsg_chem_table_size() doubles holding the
+ * mechanism (molar masses, NASA-7 fits, Arrhenius and temperature modes,
+ * nu', nu'', third-body efficiencies; layout in fsg_chem.hip), built from
+ * Air11ParkMechanism (hybrid/chemistry/air11_park.py:20-318). */
+typedef struct {
+  int32_t newton_iters;  /* 12 in ReactingFlowCoupler (hybrid/chemistry/coupling.py:30) */
+  int32_t dimensionless; /* non-dimensional flow scales (coupling.py:55-67, 135-154) */
+  double gamma;
+  double T_ref;
+  double rho_ref;
+  double e_ref;
+  double t_ref;
+} fsg_chem_params;
+
+int32_t fsg_chem_table_size(void);
+
+/* PointImplicitIntegrator.step over n cells
+ * (hybrid/chemistry/integrator.py:31-104): rho_in/rho_out (n, 11), T_in (n),
+ * dt per cell (dt_cell, may be null) or the scalar dt; T_out (n) may be null. */
+int fsg_chem_integrate(const double* table, const double* rho_in, const double* T_in,
+                       const double* dt_cell, double dt, int32_t newton_iters,
+                       int32_t constant_temperature, double* rho_out, double* T_out,
+                       int64_t n, void* stream);
+
+/* ReactingFlowCoupler.react_block (hybrid/chemistry/coupling.py:116-197) in
+ * place on U (ncells, 16); dt_elem (ncells / nodes_per_elem) may be null. */
+int fsg_chem_react(const double* table, double* U, int64_t ncells, int64_t nodes_per_elem,
+                   const double* dt_elem, double dt, const fsg_chem_params* params,
+                   void* stream);
+
+const char* fsg_chem_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,438 @@
+// Point-implicit 11-species air chemistry on gfx950 (FP64, one cell per lane).
+//
+// Restates hybrid/chemistry/integrator.py:31-104 (backward Euler on the
+// species with a fixed Newton iteration count, non-negativity clip,
+// temperature recovery at constant internal energy, total-mass rescale),
+// hybrid/chemistry/mechanism.py:159-287 (mass-action rates with detailed
+// balance, mixture energy, damped T(e) Newton) and
+// hybrid/chemistry/coupling.py:116-177 (per-node coupling to the flow state).
+// The reference obtains the Newton Jacobian with jax.jacobian; here it is the
+// analytic derivative of the same expression, including JAX's half-gradient
+// at max(x, y) ties (rho_s = 0 after the clip).  The 11x11 system is solved
+// by LU with partial pivoting held in registers (fully unrolled; the pivot
+// swap is a select over rows, so no register array is indexed dynamically).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/fluid_gfx950.h"
+
+namespace fsgchem {
+
+constexpr int NS = 11;
+constexpr int NR = 22;
+constexpr double R_UNIV = 8.31446261815324;
+constexpr double P_STD = 101325.0;
+
+// Mechanism table layout (doubles), written by the host:
+//   [0, 11)              molar masses
+//   [11, 11 + 17*11)     NASA-7 per species: t_low, t_mid, t_high, low[7], high[7]
+//   then per reaction 6: A, b, theta, third_body, fwd_mode, rev_mode
+//   then nu' (22x11), nu'' (22x11), alpha (22x11)
+constexpr int OFF_NASA = NS;
+constexpr int OFF_RX = OFF_NASA + 17 * NS;
+constexpr int OFF_NU1 = OFF_RX + 6 * NR;
+constexpr int OFF_NU2 = OFF_NU1 + NR * NS;
+constexpr int OFF_ALPHA = OFF_NU2 + NR * NS;
+constexpr int TABLE_SIZE = OFF_ALPHA + NR * NS;
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double clampd(double x, double lo, double hi) { return dmin(dmax(x, lo), hi); }
+// d max(x, y) / dx with the half gradient at a tie.
+__device__ __forceinline__ double dmax_dx(double x, double y) {
+  return x > y ? 1.0 : (x == y ? 0.5 : 0.0);
+}
+
+// NASA-7 with T clamped to [t_low, t_high] inside the polynomial
+// (hybrid/chemistry/nasa.py:47-119).
+struct Thermo {
+  double cp, h, s;  // cp/R, h/(RT), s/R
+};
+
+__device__ __forceinline__ Thermo nasa(const double* c, double T) {
+  double t = clampd(T, c[0], c[2]);
+  const double* a = t < c[1] ? c + 3 : c + 10;
+  Thermo r;
+  r.cp = a[0] + a[1] * t + a[2] * (t * t) + a[3] * (t * t * t) + a[4] * (t * t * t * t);
+  r.h = a[0] + a[1] * t / 2.0 + a[2] * (t * t) / 3.0 + a[3] * (t * t * t) / 4.0 +
+        a[4] * (t * t * t * t) / 5.0 + a[5] / t;
+  r.s = a[0] * log(t) + a[1] * t + a[2] * (t * t) / 2.0 + a[3] * (t * t * t) / 3.0 +
+        a[4] * (t * t * t * t) / 4.0 + a[6];
+  return r;
+}
+
+__device__ __forceinline__ double select_temperature(int mode, double T, double Tv) {
+  double Ta = sqrt(dmax(T * Tv, 1.0));
+  return mode == 1 ? Ta : (mode == 2 ? Tv : T);
+}
+
+// e_mix(T) with h = h/RT * R T / M_s (T unclamped outside), as
+// hybrid/chemistry/mechanism.py:222-260.
+__device__ __forceinline__ double mixture_energy(const double* tab, const double* Y, double T) {
+  double e = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Thermo th = nasa(tab + OFF_NASA + 17 * s, T);
+    e += Y[s] * ((th.h - 1.0) * (R_UNIV * T / tab[s]));
+  }
+  return e;
+}
+
+// hybrid/chemistry/mechanism.py:262-287
+__device__ __forceinline__ double recover_temperature(const double* tab, const double* rho, double e_target,
+                                      double T, int iters) {
+  double tot = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) tot += rho[s];
+  tot = dmax(tot, 1.0e-14);
+  double Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Y[s] = dmax(rho[s], 0.0) / tot;
+  for (int it = 0; it < iters; ++it) {
+    double Ts = clampd(T, 50.0, 50000.0);
+    double e = 0.0, cv = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      Thermo th = nasa(tab + OFF_NASA + 17 * s, Ts);
+      e += Y[s] * ((th.h - 1.0) * (R_UNIV * Ts / tab[s]));
+      cv += Y[s] * ((th.cp - 1.0) * (R_UNIV / tab[s]));
+    }
+    double dT = -(e - e_target) / dmax(cv, 1.0e-5);
+    T = clampd(T + clampd(dT, -0.5 * T, 0.5 * T), 100.0, 50000.0);
+  }
+  return T;
+}
+
+// Net mass production rates w and d w / d rho at fixed (T, Tv)
+// (hybrid/chemistry/mechanism.py:159-220).
+__device__ __forceinline__ void rates(const double* tab, const double* rho, double T, double Tv, double* w,
+                      double (*dw)[NS]) {
+  const double* mw = tab;
+  const double* rx = tab + OFF_RX;
+  double C[NS], dC[NS], lc[NS], dlc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double x = rho[s] / mw[s];
+    C[s] = dmax(x, 0.0);
+    dC[s] = dmax_dx(x, 0.0) / mw[s];
+    double cs = dmax(C[s], 1.0e-100);
+    lc[s] = log(cs);
+    dlc[s] = dmax_dx(C[s], 1.0e-100) / cs;
+  }
+  // Gibbs energies at T and at Tv (the reverse temperatures used here).
+  double gT[NS], gV[NS];
+  const double tT = dmax(T, 10.0), tV = dmax(Tv, 10.0);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Thermo a = nasa(tab + OFF_NASA + 17 * s, tT);
+    gT[s] = a.h - a.s;
+    if (Tv == T) {
+      gV[s] = gT[s];
+    } else {
+      Thermo b = nasa(tab + OFF_NASA + 17 * s, tV);
+      gV[s] = b.h - b.s;
+    }
+  }
+  // dw accumulates M_s d omega_s / d rho_j directly (no separate omega
+  // Jacobian held in registers).
+  double om[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    om[s] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) dw[s][j] = 0.0;
+  }
+  for (int r = 0; r < NR; ++r) {
+    const double A = rx[6 * r + 0], b = rx[6 * r + 1], th = rx[6 * r + 2];
+    const bool third = rx[6 * r + 3] != 0.0;
+    const int fm = (int)rx[6 * r + 4], rm = (int)rx[6 * r + 5];
+    const double* n1 = tab + OFF_NU1 + r * NS;
+    const double* n2 = tab + OFF_NU2 + r * NS;
+    const double* al = tab + OFF_ALPHA + r * NS;
+    double tf = dmax(select_temperature(fm, T, Tv), 10.0);
+    double kf = A * pow(tf, b) * exp(-th / tf);
+    double tr = dmax(select_temperature(rm, T, Tv), 10.0);
+    double dG = 0.0, dn = 0.0;
+    // reverse modes are T or Tv in this mechanism; Ta falls back to a fresh
+    // Gibbs evaluation at that temperature.
+    if (rm == 1) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        Thermo a = nasa(tab + OFF_NASA + 17 * s, tr);
+        dG += (n2[s] - n1[s]) * (a.h - a.s);
+      }
+    } else {
+      const double* g = rm == 2 ? gV : gT;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) dG += (n2[s] - n1[s]) * g[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) dn += n2[s] - n1[s];
+    double Kc = exp(-clampd(dG, -700.0, 700.0)) * pow(P_STD / (R_UNIV * tr), dn);
+    double kr = kf / dmax(Kc, 1.0e-300);
+    double lf = 0.0, lr = 0.0, Meff = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      lf += n1[s] * lc[s];
+      lr += n2[s] * lc[s];
+      Meff += al[s] * C[s];
+    }
+    double fwd = exp(clampd(lf, -700.0, 700.0));
+    double rev = exp(clampd(lr, -700.0, 700.0));
+    double cf = fabs(lf) < 700.0 ? 1.0 : (fabs(lf) == 700.0 ? 0.5 : 0.0);
+    double cr = fabs(lr) < 700.0 ? 1.0 : (fabs(lr) == 700.0 ? 0.5 : 0.0);
+    double col = third ? Meff : 1.0;
+    double q = kf * fwd * col - kr * rev * col;
+    double dq[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double dfw = fwd * cf * n1[s] * dlc[s];
+      double drv = rev * cr * n2[s] * dlc[s];
+      double dcol = third ? al[s] : 0.0;
+      dq[s] = (kf * (dfw * col + fwd * dcol) - kr * (drv * col + rev * dcol)) * dC[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double d = n2[s] - n1[s];
+      if (d != 0.0) {
+        om[s] += d * q;
+        const double md = mw[s] * d;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) dw[s][j] += md * dq[j];
+      }
+    }
+  }
+  // w = M omega, then the exact mass-sum correction w - Y sum(w).
+  double srho = 0.0, S = 0.0;
+  double wr[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    srho += rho[s];
+    wr[s] = om[s] * mw[s];
+    S += wr[s];
+  }
+  double tot = dmax(srho, 1.0e-14);
+  double dtot = dmax_dx(srho, 1.0e-14);
+  double colsum[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    colsum[j] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) colsum[j] += dw[s][j];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double Y = dmax(rho[s], 0.0) / tot;
+    w[s] = wr[s] - Y * S;
+    double dpos = dmax_dx(rho[s], 0.0);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      double dY = (s == j ? dpos / tot : 0.0) - dmax(rho[s], 0.0) / (tot * tot) * dtot;
+      dw[s][j] = dw[s][j] - dY * S - Y * colsum[j];
+    }
+  }
+}
+
+// Solve A x = b (A overwritten) by LU with partial pivoting, all unrolled.
+__device__ __forceinline__ void solve(double (*A)[NS], double* b) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    int piv = k;
+    double best = fabs(A[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < NS; ++i) {
+      double v = fabs(A[i][k]);
+      if (v > best) {
+        best = v;
+        piv = i;
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < NS; ++i) {
+      bool sw = (piv == i);
+#pragma unroll
+      for (int j = k; j < NS; ++j) {
+        double t = A[k][j];
+        A[k][j] = sw ? A[i][j] : t;
+        A[i][j] = sw ? t : A[i][j];
+      }
+      double t = b[k];
+      b[k] = sw ? b[i] : t;
+      b[i] = sw ? t : b[i];
+    }
+#pragma unroll
+    for (int i = k + 1; i < NS; ++i) {
+      double f = A[i][k] / A[k][k];
+#pragma unroll
+      for (int j = k + 1; j < NS; ++j) A[i][j] -= f * A[k][j];
+      b[i] -= f * b[k];
+    }
+  }
+#pragma unroll
+  for (int k = NS - 1; k >= 0; --k) {
+    double acc = b[k];
+#pragma unroll
+    for (int j = k + 1; j < NS; ++j) acc -= A[k][j] * b[j];
+    b[k] = acc / A[k][k];
+  }
+}
+
+// hybrid/chemistry/integrator.py:31-104
+__device__ __forceinline__ double integrate(const double* tab, double* rho, double T, double dt, int iters,
+                            int constant_temperature) {
+  double rho0[NS], Y0[NS];
+  double total = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rho0[s] = dmax(rho[s], 1.0e-20);
+    total += rho0[s];
+  }
+  {
+    double tot = dmax(total, 1.0e-14);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) Y0[s] = dmax(rho0[s], 0.0) / tot;
+  }
+  const double e_init = mixture_energy(tab, Y0, T);
+  double r[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) r[s] = rho0[s];
+  double Tk = T, Tvk = T;
+  for (int it = 0; it < iters; ++it) {
+    double w[NS];
+    double J[NS][NS];
+    rates(tab, r, Tk, Tvk, w, J);
+    double rhs[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      rhs[s] = -(r[s] - rho0[s] - dt * w[s]);
+#pragma unroll
+      for (int j = 0; j < NS; ++j) J[s][j] = (s == j ? 1.0 : 0.0) - dt * J[s][j];
+    }
+    solve(J, rhs);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) r[s] = dmax(r[s] + rhs[s], 0.0);
+    if (!constant_temperature) {
+      Tk = recover_temperature(tab, r, e_init, Tk, 10);
+      Tvk = Tk;
+    }
+  }
+  double sr = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) sr += r[s];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rho[s] = r[s] * (total / sr);
+  return Tk;
+}
+
+__global__ __launch_bounds__(64) void integrate_kernel(const double* __restrict__ tab,
+                                                       const double* __restrict__ rho_in,
+                                                       const double* __restrict__ T_in,
+                                                       const double* __restrict__ dt_cell,
+                                                       double dt, int iters, int const_T,
+                                                       double* __restrict__ rho_out,
+                                                       double* __restrict__ T_out, int64_t n) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double r[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) r[s] = rho_in[c * NS + s];
+  double h = dt_cell ? dt_cell[c] : dt;
+  double T = integrate(tab, r, T_in[c], h, iters, const_T);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rho_out[c * NS + s] = r[s];
+  if (T_out) T_out[c] = T;
+}
+
+// hybrid/chemistry/coupling.py:116-177 on a (cells, 16) state, in place.
+__global__ __launch_bounds__(64) void react_kernel(const double* __restrict__ tab,
+                                                   double* __restrict__ U, int64_t n,
+                                                   int64_t nodes_per_elem,
+                                                   const double* __restrict__ dt_elem, double dt,
+                                                   fsg_chem_params p) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double* u = U + c * (5 + NS);
+  double h = dt_elem ? dt_elem[c / nodes_per_elem] : dt;
+  double rt = dmax(u[0], 1.0e-14);
+  double rs[NS], sum = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rs[s] = dmax(u[5 + s], 0.0);
+    sum += rs[s];
+  }
+  double scale = rt / dmax(sum, 1.0e-20);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rs[s] *= scale;
+  double ek = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / (rt * rt);
+  if (p.dimensionless) {
+    double eth_s = dmax(u[4] / rt - ek, 1.0e-4);
+    double eth = eth_s * p.e_ref;
+    double rp[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) rp[s] = rs[s] * p.rho_ref;
+    double Tg = dmax((p.gamma - 1.0) * eth_s * p.T_ref, 200.0);
+    double T0 = recover_temperature(tab, rp, eth, Tg, 8);
+    integrate(tab, rp, T0, h * p.t_ref, p.newton_iters, 0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) rs[s] = rp[s] / p.rho_ref;
+  } else {
+    double eth = dmax(u[4] / rt - ek, 1.0e3);
+    double T0 = recover_temperature(tab, rs, eth, 1000.0, 8);
+    integrate(tab, rs, T0, h, p.newton_iters, 0);
+  }
+  double s2 = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rs[s] = dmax(rs[s], 0.0);
+    s2 += rs[s];
+  }
+  double f = rt / dmax(s2, 1.0e-20);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) u[5 + s] = rs[s] * f;
+}
+
+}  // namespace fsgchem
+
+static thread_local std::string g_chem_error;
+
+static int chem_fail(int code, const char* msg) {
+  g_chem_error = msg;
+  return code;
+}
+
+extern "C" {
+
+int32_t fsg_chem_table_size(void) { return fsgchem::TABLE_SIZE; }
+
+int fsg_chem_integrate(const double* table, const double* rho_in, const double* T_in,
+                       const double* dt_cell, double dt, int32_t newton_iters,
+                       int32_t constant_temperature, double* rho_out, double* T_out, int64_t n,
+                       void* stream) {
+  if (!table || !rho_in || !T_in || !rho_out) return chem_fail(-10, "null argument");
+  if (n <= 0) return 0;
+  if (newton_iters < 0) return chem_fail(-11, "negative iteration count");
+  unsigned grid = (unsigned)((n + 63) / 64);
+  hipLaunchKernelGGL(fsgchem::integrate_kernel, dim3(grid), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), table, rho_in, T_in, dt_cell, dt,
+                     newton_iters, constant_temperature, rho_out, T_out, n);
+  return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
+}
+
+int fsg_chem_react(const double* table, double* U, int64_t ncells, int64_t nodes_per_elem,
+                   const double* dt_elem, double dt, const fsg_chem_params* params,
+                   void* stream) {
+  if (!table || !U || !params) return chem_fail(-10, "null argument");
+  if (ncells <= 0) return 0;
+  if (nodes_per_elem <= 0) return chem_fail(-12, "nodes_per_elem must be positive");
+  unsigned grid = (unsigned)((ncells + 63) / 64);
+  hipLaunchKernelGGL(fsgchem::react_kernel, dim3(grid), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), table, U, ncells, nodes_per_elem, dt_elem,
+                     dt, *params);
+  return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
+}
+
+const char* fsg_chem_last_error(void) { return g_chem_error.c_str(); }
+
+}  // extern "C"

@@ -3,6 +3,11 @@
 // Each function restates one function of the reference's hybrid/solver.py and
 // hybrid/boundary.py (cited per function) for one quadrature point held in
 // registers.  ``real`` is double on the plain path and float on the FP32 path.
+//
+// The conversion and Euler flux functions disable floating-point contraction:
+// the residual subtracts F(a) from F(U) with U == a bitwise in uniform
+// regions (the anchor of hybrid/spatial.py:41-70), which cancels exactly
+// only if every inlined copy rounds identically.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -55,6 +60,7 @@ struct Prim {
 
 template <int V, typename real>
 __device__ __forceinline__ Prim<V, real> c2p(const real* U, real gamma) {
+#pragma clang fp contract(off)
   Prim<V, real> w;
   w.rho = U[0];
   w.safe = rmax(U[0], real(1.0e-14));
@@ -69,6 +75,7 @@ __device__ __forceinline__ Prim<V, real> c2p(const real* U, real gamma) {
 // hybrid/solver.py:63-81 from primitive [rho, vel, p, Y...].
 template <int V, typename real>
 __device__ __forceinline__ void p2c(const real* W, real gamma, real* U) {
+#pragma clang fp contract(off)
   real rho = W[0];
   U[0] = rho;
   U[1] = rho * W[1];
@@ -81,6 +88,7 @@ __device__ __forceinline__ void p2c(const real* W, real gamma, real* U) {
 
 template <int V, typename real>
 __device__ __forceinline__ void c2p_full(const real* U, real gamma, real* W) {
+#pragma clang fp contract(off)
   Prim<V, real> w = c2p<V, real>(U, gamma);
   W[0] = w.rho;
   W[1] = w.vel[0];
@@ -95,6 +103,7 @@ __device__ __forceinline__ void c2p_full(const real* U, real gamma, real* W) {
 template <int V, typename real>
 __device__ __forceinline__ void normal_flux(const real* U, const Prim<V, real>& w, const real* n,
                                             real* F) {
+#pragma clang fp contract(off)
   real un = w.vel[0] * n[0] + w.vel[1] * n[1] + w.vel[2] * n[2];
   F[0] = w.rho * un;
   F[1] = w.rho * w.vel[0] * un + w.p * n[0];
@@ -108,6 +117,7 @@ __device__ __forceinline__ void normal_flux(const real* U, const Prim<V, real>& 
 // hybrid/solver.py:124-130: Cartesian flux along axis d.
 template <int V, typename real>
 __device__ __forceinline__ void axis_flux(const real* U, const Prim<V, real>& w, int d, real* F) {
+#pragma clang fp contract(off)
   real un = w.vel[d];
   F[0] = w.rho * un;
   F[1] = w.rho * w.vel[0] * un + (d == 0 ? w.p : real(0));
@@ -129,6 +139,7 @@ __device__ __forceinline__ real safe_signed(real x) {
 template <int V, typename real>
 __device__ __forceinline__ void rusanov(const real* UL, const real* UR, const real* n, real gamma,
                                         real* F) {
+#pragma clang fp contract(off)
   Prim<V, real> wl = c2p<V, real>(UL, gamma), wr = c2p<V, real>(UR, gamma);
   real unl = wl.vel[0] * n[0] + wl.vel[1] * n[1] + wl.vel[2] * n[2];
   real unr = wr.vel[0] * n[0] + wr.vel[1] * n[1] + wr.vel[2] * n[2];
@@ -147,6 +158,7 @@ __device__ __forceinline__ void rusanov(const real* UL, const real* UR, const re
 template <int V, typename real>
 __device__ __forceinline__ void hllc_side(const real* U, const Prim<V, real>& w, const real* n,
                                           real S, real SM, const real* F, real* Fs) {
+#pragma clang fp contract(off)
   real un = w.vel[0] * n[0] + w.vel[1] * n[1] + w.vel[2] * n[2];
   real den = safe_signed(S - SM);
   real comp = (S - un) / den;
@@ -169,6 +181,7 @@ __device__ __forceinline__ void hllc_side(const real* U, const Prim<V, real>& w,
 template <int V, typename real>
 __device__ __forceinline__ void hllc(const real* UL, const real* UR, const real* n, real gamma,
                                      real* F) {
+#pragma clang fp contract(off)
   Prim<V, real> wl = c2p<V, real>(UL, gamma), wr = c2p<V, real>(UR, gamma);
   real FL[V];
   normal_flux<V, real>(UL, wl, n, FL);

@@ -42,6 +42,13 @@ class GroupDesc(ctypes.Structure):
         ("out_owner", ctypes.c_int64), ("out_neighbor", ctypes.c_int64)]
 
 
+class ChemParams(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_int32), ("dimensionless", ctypes.c_int32),
+                ("gamma", ctypes.c_double), ("T_ref", ctypes.c_double),
+                ("rho_ref", ctypes.c_double), ("e_ref", ctypes.c_double),
+                ("t_ref", ctypes.c_double)]
+
+
 class Boundary(ctypes.Structure):
     _fields_ = [("has_boundary", ctypes.c_int32), ("has_wall_temperature", ctypes.c_int32),
                 ("wall_temperature", ctypes.c_double), ("farfield", ctypes.c_double * 16)]
@@ -79,6 +86,12 @@ def lib():
         L.fsg_last_error.argtypes = []
         L.fsg_last_error.restype = ctypes.c_char_p
         L.fsg_abi_version.restype = ctypes.c_int32
+        L.fsg_chem_table_size.restype = ctypes.c_int32
+        L.fsg_chem_integrate.argtypes = [vp, vp, vp, vp, f64, i32, i32, vp, vp, i64, vp]
+        L.fsg_chem_integrate.restype = ctypes.c_int
+        L.fsg_chem_react.argtypes = [vp, vp, i64, i64, vp, f64, ctypes.POINTER(ChemParams), vp]
+        L.fsg_chem_react.restype = ctypes.c_int
+        L.fsg_chem_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -86,6 +99,12 @@ def lib():
 def check(rc):
     if rc != 0:
         raise RuntimeError(f"libfluid_gfx950 error {rc}: {lib().fsg_last_error().decode()}")
+
+
+def check_chem(rc):
+    if rc != 0:
+        raise RuntimeError(f"libfluid_gfx950 chemistry error {rc}: "
+                           f"{lib().fsg_chem_last_error().decode()}")
 
 
 def ptr(t):

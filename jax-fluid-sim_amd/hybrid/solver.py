@@ -67,13 +67,17 @@ class HybridCompressibleSolver:
         self.thermal_conductivity = (self.dynamic_viscosity * self.specific_heat_cp
                                      / self.prandtl)
         self.dtype = dtype
-        self.device = torch.device(device)
+        dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
 
     # -- state helpers (hybrid/solver.py:63-98, 421-470) -------------------
     def primitive_to_conserved(self, W):
         rho = W[..., 0:1]
         vel = W[..., 1:4]
-        E = W[..., 4:5] / (self.gamma - 1.0) + 0.5 * rho * (vel ** 2).sum(-1, keepdim=True)
+        ke = (vel * vel).sum(-1)[..., None]
+        E = W[..., 4:5] / (self.gamma - 1.0) + 0.5 * rho * ke
         parts = [rho, rho * vel, E]
         if self.layout.species_names:
             parts.append(rho * W[..., 5:])
@@ -83,7 +87,7 @@ class HybridCompressibleSolver:
         rho = U[..., 0:1]
         safe = rho.clamp_min(1e-14) if torch.is_tensor(U) else np.maximum(rho, 1e-14)
         vel = U[..., 1:4] / safe
-        ke = 0.5 * (U[..., 1:4] ** 2).sum(-1, keepdims=True) / safe
+        ke = 0.5 * (U[..., 1:4] * U[..., 1:4]).sum(-1)[..., None] / safe
         p = (self.gamma - 1.0) * (U[..., 4:5] - ke)
         parts = [rho, vel, p]
         if self.layout.species_names:

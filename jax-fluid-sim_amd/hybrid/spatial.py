@@ -55,7 +55,8 @@ class HybridEulerOperator:
         ts = []
         for t in self.block_types:
             U = states[t]
-            if U.dtype != self.solver.dtype or not U.is_contiguous() or U.device != self.solver.device:
+            if (U.dtype != self.solver.dtype or not U.is_contiguous()
+                    or U.device != self.solver.device):
                 raise ValueError(f"{t} state must be a contiguous {self.solver.dtype} device tensor")
             ts.append(U)
         return ts
