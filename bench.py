@@ -51,6 +51,16 @@ def element_flops(N, Q, V):
             + 2 * N * 3 * Q * V + 2 * N * N * V)
 
 
+def measured_traffic(kernel, key):
+    """Calibrated HBM bytes per launch from the committed PMC summary (32^3, p=3)."""
+    import re
+    path = ROOT / "profiles" / "hbm_traffic_tgv32_p3.md"
+    if not path.exists():
+        return None
+    m = re.search(rf"traffic_bytes_per_launch {kernel} {key} = (\d+)", path.read_text())
+    return int(m.group(1)) if m else None
+
+
 def run_path(topo, dtype, steps, warmup, dt):
     import torch
     from hybrid import HybridCompressibleSolver, HybridNavierStokesOperator, HybridSSPRK3
@@ -86,6 +96,32 @@ def run_path(topo, dtype, steps, warmup, dt):
     return dict(state=U, elapsed=elapsed, dt=dt, valid=bool(valid), rho_min=float(rmin),
                 p_min=float(pmin), elem_ms=bms[0] / max(bn[0], 1), elem_launches=bn[0],
                 face_ms=gms[0] / max(gn[0], 1), face_launches=gn[0])
+
+
+def chemistry_rate(cells, seed=11):
+    """0-D air-11 batch: 2^20 cells, T in [3000, 12000] K, p log-uniform in
+    [1e3, 1e5] Pa, Dirichlet(2) mass fractions made charge-neutral, dt 1e-7 s;
+    plain FP64 point-implicit step (12 Newton iterations, T recovered at
+    constant internal energy) on the GPU."""
+    import torch
+    from hybrid.chemistry import Air11ParkMechanism, PointImplicitIntegrator
+    from hybrid.chemistry.batch import make_batch
+    rho, T = make_batch(cells, seed=seed)
+    mech = Air11ParkMechanism()
+    integ = PointImplicitIntegrator(mech, max_newton_iters=12)
+    r = torch.as_tensor(rho, device="cuda")
+    t = torch.as_tensor(T, device="cuda")
+    integ.step(r[:4096], t[:4096], 1e-7, constant_temperature=False)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    out, _, _ = integ.step(r, t, 1e-7, constant_temperature=False)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    ok = bool(torch.isfinite(out).all()) and bool((out >= 0).all())
+    return {"cells": cells, "plain_fp64_ms": ms, "plain_fp64_cell_updates_per_s": cells / (ms * 1e-3),
+            "finite_nonnegative": ok}
 
 
 def cpu_baseline():
@@ -131,6 +167,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=32, help="elements per box edge")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-chem", action="store_true")
     ap.add_argument("--only", choices=["f64", "f32"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -171,10 +208,7 @@ def main():
     r = res[key]
     flops = element_flops(N, Q, V) * E
     achieved = flops / (r["elem_ms"] * 1e-3) / 1e12
-    traffic = None
-    tfile = ROOT / "profiles" / f"element_kernel_traffic_{key}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    traffic = measured_traffic("element", key) if args.n == 32 else None
     out = {
         "metric": "DOF-updates/s per RK stage, plain FP64 vs FP32+surrogate chem, at fixed L2 error",
         "value": r["rate"], "unit": "DOF-updates/s per RK stage", "n_gpus": 1,
@@ -195,6 +229,8 @@ def main():
                      "flops_per_launch": flops, "avg_launch_ms": r["elem_ms"]},
         "face_kernel_avg_ms": r["face_ms"],
     }
+    if not args.skip_chem:
+        out["chemistry"] = chemistry_rate(1 << 20)
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
