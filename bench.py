@@ -98,30 +98,82 @@ def run_path(topo, dtype, steps, warmup, dt):
                 face_ms=gms[0] / max(gn[0], 1), face_launches=gn[0])
 
 
-def chemistry_rate(cells, seed=11):
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    out = fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    return out, ev0.elapsed_time(ev1)
+
+
+def chemistry_rate(cells, seed=11, train_cells=1 << 18, train_steps=3000):
     """0-D air-11 batch: 2^20 cells, T in [3000, 12000] K, p log-uniform in
-    [1e3, 1e5] Pa, Dirichlet(2) mass fractions made charge-neutral, dt 1e-7 s;
-    plain FP64 point-implicit step (12 Newton iterations, T recovered at
-    constant internal energy) on the GPU."""
+    [1e3, 1e5] Pa, Dirichlet(2) mass fractions made charge-neutral, dt 1e-7 s.
+    Plain FP64 point-implicit step (12 Newton iterations, T recovered at
+    constant internal energy) and the BF16-MFMA surrogate (trained here on
+    2^18 integrator results from another seed) with its integrator fallback."""
     import torch
     from hybrid.chemistry import Air11ParkMechanism, PointImplicitIntegrator
     from hybrid.chemistry.batch import make_batch
+    from hybrid.chemistry.surrogate import ChemistrySurrogate
     rho, T = make_batch(cells, seed=seed)
     mech = Air11ParkMechanism()
     integ = PointImplicitIntegrator(mech, max_newton_iters=12)
     r = torch.as_tensor(rho, device="cuda")
     t = torch.as_tensor(T, device="cuda")
     integ.step(r[:4096], t[:4096], 1e-7, constant_temperature=False)
+    ref, ms = _timed(lambda: integ.step(r, t, 1e-7, constant_temperature=False)[0])
+    ok = bool(torch.isfinite(ref).all()) and bool((ref >= 0).all())
+    out = {"cells": cells, "plain_fp64_ms": ms,
+           "plain_fp64_cell_updates_per_s": cells / (ms * 1e-3), "finite_nonnegative": ok}
+    print(f"[bench] chemistry plain FP64: {ms:.3f} ms", file=sys.stderr, flush=True)
+
+    # Surrogate of the isothermal substep: data from the integrator itself
+    # (initial states and one step along their trajectories), FP32 training
+    # kernels, BF16 MFMA inference.
+    (ref_ct, ms_ct) = _timed(lambda: integ.step(r, t, 1e-7, constant_temperature=True)[0])
+    out["plain_fp64_isothermal_ms"] = ms_ct
+    rho_tr, T_tr = make_batch(train_cells, seed=seed + 1)
+    sur = ChemistrySurrogate(integ, constant_temperature=True, seed=0)
+    t0 = time.perf_counter()
+    X, O, _ = sur.dataset(rho_tr, T_tr, 1e-7, trajectory_steps=2)
+    sur.set_normalisation(X, O)
+    hist = sur.train(X, O, steps=train_steps, batch=4096)
     torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    out, _, _ = integ.step(r, t, 1e-7, constant_temperature=False)
-    ev1.record()
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
-    ok = bool(torch.isfinite(out).all()) and bool((out >= 0).all())
-    return {"cells": cells, "plain_fp64_ms": ms, "plain_fp64_cell_updates_per_s": cells / (ms * 1e-3),
-            "finite_nonnegative": ok}
+    train_s = time.perf_counter() - t0
+    print(f"[bench] surrogate trained in {train_s:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}",
+          file=sys.stderr, flush=True)
+    sur.infer(r[:4096], t[:4096], 1e-7)
+    (pred, fb), ms_inf = _timed(lambda: sur.infer(r, t, 1e-7))
+    (full, _, _), ms_step = _timed(lambda: sur.step(r, t, 1e-7))
+    Yref = ref_ct / ref_ct.sum(1, keepdim=True)
+    l1 = lambda x: float((x / x.sum(1, keepdim=True) - Yref).abs().sum(1).mean())
+    # Sensitivity of the coupled-temperature integrator to ~1 ulp of input noise.
+    pert = r * (1 + 2e-16 * torch.randn(r.shape, generator=torch.Generator(device="cuda").manual_seed(1),
+                                        device="cuda", dtype=torch.float64))
+    ref2 = integ.step(pert, t, 1e-7, constant_temperature=False)[0]
+    Yc = ref / ref.sum(1, keepdim=True)
+    l1_c = lambda x: float((x / x.sum(1, keepdim=True) - Yc).abs().sum(1).mean())
+    out["surrogate"] = {
+        "train_cells": train_cells, "train_steps": train_steps, "train_s": train_s,
+        "train_loss_first_last": [hist[0], hist[-1]],
+        "infer_ms": ms_inf, "infer_cell_updates_per_s": cells / (ms_inf * 1e-3),
+        "step_with_fallback_ms": ms_step,
+        "step_cell_updates_per_s": cells / (ms_step * 1e-3),
+        "speedup_vs_plain_fp64_isothermal": ms_ct / ms_step,
+        "fallback_fraction": float(fb.float().mean()),
+        "mean_L1_massfraction_error_vs_integrator": l1(full),
+        "mean_L1_identity_predictor": l1(r),
+        "target": "isothermal point-implicit substep (12 Newton iterations), dt 1e-7 s",
+        "coupled_T_integrator_1ulp_sensitivity_mean_L1": l1_c(ref2),
+    }
+    print(f"[bench] surrogate: infer {ms_inf:.3f} ms, step+fallback {ms_step:.3f} ms, "
+          f"fallback {out['surrogate']['fallback_fraction']:.4f}, L1 {out['surrogate']['mean_L1_massfraction_error_vs_integrator']:.4f}",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline():

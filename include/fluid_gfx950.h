@@ -203,6 +203,45 @@ int fsg_chem_react(const double* table, double* U, int64_t ncells, int64_t nodes
 
 const char* fsg_chem_last_error(void);
 
+/* ---- Chemistry surrogate (MLP 14 -> 128 -> 128 -> 128 -> 11) --------------
+ * No counterpart in the reference: a learned replacement for the stiff
+ * substep of PointImplicitIntegrator.step (hybrid/chemistry/integrator.py:31),
+ * trained on that integrator's own results.  All training buffers are FP32,
+ * row-major; W is (N out, K in) like torch.nn.Linear. */
+
+/* Z = X W^T + b (M x N), H = silu(Z) (or Z when silu_on == 0). */
+int fsg_mlp_linear_forward(const float* X, const float* W, const float* b, float* Z, float* H,
+                           int32_t M, int32_t N, int32_t K, int32_t silu_on, void* stream);
+/* Backward of the layer above given dH (M x N, overwritten with dZ):
+ * dW = dZ^T X (N x K), db = column sums of dZ (N), dX = dZ W (M x K; may be
+ * null for the first layer). */
+int fsg_mlp_linear_backward(float* dH, const float* Z, const float* X, const float* W, float* dW,
+                            float* db, float* dX, int32_t M, int32_t N, int32_t K,
+                            int32_t silu_on, void* stream);
+/* g = scale * (O - T) over n entries (scale = 2 / n for the mean squared error). */
+int fsg_mlp_mse_grad(const float* O, const float* T, float* g, int64_t n, float scale,
+                     void* stream);
+/* One Adam step (Kingma & Ba 2015, bias-corrected; step counts from 1). */
+int fsg_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                 float b2, float eps, int32_t step, void* stream);
+
+/* Inference on BF16 MFMA: packed_weights holds fsg_mlp_packed_weight_count()
+ * bf16 values in the lane order of the matrix-core A operand (written by
+ * hybrid/chemistry/surrogate.py), biases 3*128 + 16 floats, consts
+ * fsg_mlp_consts_size() bytes (feature/target normalisation, training hull,
+ * projection tolerance, species data).  For each of n cells (rho_in (n, 11),
+ * T_in (n), dt per cell or scalar) writes the predicted species densities,
+ * re-projected onto the input's N, O, charge and mass content, into rho_out;
+ * cells outside the hull or whose projection misses the tolerance get
+ * fallback[c] = 1 and rho_out = rho_in, to be advanced by fsg_chem_integrate.
+ * blocks <= 0 launches one block per 64 cells. */
+int32_t fsg_mlp_packed_weight_count(void);
+int32_t fsg_mlp_consts_size(void);
+int fsg_mlp_infer(const void* packed_weights, const float* biases, const void* consts,
+                  const double* rho_in, const double* T_in, const double* dt_cell, double dt,
+                  double* rho_out, int32_t* fallback, int64_t n, int32_t blocks, void* stream);
+const char* fsg_mlp_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,6 +142,7 @@ constexpr int HID = 128;
 constexpr int NIN = 14;
 constexpr int NOUT = 11;
 constexpr int K1 = 32;           // padded input width
+constexpr int PROJ_ITERS = 32;   // damped dual Newton iterations, at most
 // Packed bf16 weights (A-operand order, see pack_weights in the host code):
 //   W1: 8 out-tiles x 1 k-step, W2/W3: 8 x 4, W4: 1 x 4 ; each (tile, step)
 //   block holds 64 lanes x 8 bf16.
@@ -309,74 +310,89 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
       }
 #pragma unroll
       for (int s = 0; s < NOUT; ++s) Yp[s] /= z;
-      double A[4][NOUT], cvec[4];
+      // Rows: N atoms, O atoms, charge (per unit mass).  Total mass is not a
+      // fourth row: the species table gives ions the neutral molar mass, so
+      // sum(Y) = M_N * (N row) + M_O * (O row) + Y_e identically and the four
+      // would freeze Y_e.  After the projection the total mass is restored by
+      // the integrator's own final rescale (hybrid/chemistry/integrator.py:102),
+      // which keeps the charge exact and moves N and O by the change of Y_e,
+      // as the integrator itself does.
+      double A[3][NOUT], cvec[3];
 #pragma unroll
       for (int s = 0; s < NOUT; ++s) {
         A[0][s] = cst->n_atoms[s] / cst->molar_mass[s];
         A[1][s] = cst->o_atoms[s] / cst->molar_mass[s];
         A[2][s] = cst->charge[s] / cst->molar_mass[s];
-        A[3][s] = 1.0;
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < 3; ++i) {
         cvec[i] = 0.0;
 #pragma unroll
         for (int s = 0; s < NOUT; ++s) cvec[i] += A[i][s] * Y0[s];
       }
-      double lam[4] = {0, 0, 0, 0};
+      // Newton on the dual with the step limited to a change of 2 in any
+      // species' log (the electron column of A is ~1e6 times the others, so
+      // a full step from lam = 0 can overflow exp); stops when converged.
+      double lam[3] = {0, 0, 0};
       double res = 0.0;
-      for (int it = 0; it < 12; ++it) {
+      for (int it = 0; it < PROJ_ITERS; ++it) {
         double Yq[NOUT];
 #pragma unroll
-        for (int s = 0; s < NOUT; ++s) {
-          double e = A[0][s] * lam[0] + A[1][s] * lam[1] + A[2][s] * lam[2] + lam[3];
-          Yq[s] = Yp[s] * exp(e);
-        }
-        double rr[4], J[4][4];
+        for (int s = 0; s < NOUT; ++s)
+          Yq[s] = Yp[s] * exp(A[0][s] * lam[0] + A[1][s] * lam[1] + A[2][s] * lam[2]);
+        double rr[3], J[3][3];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 3; ++i) {
           rr[i] = -cvec[i];
 #pragma unroll
           for (int s = 0; s < NOUT; ++s) rr[i] += A[i][s] * Yq[s];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
+          for (int k = 0; k < 3; ++k) {
             double acc = 0.0;
 #pragma unroll
             for (int s = 0; s < NOUT; ++s) acc += A[i][s] * Yq[s] * A[k][s];
             J[i][k] = acc;
           }
         }
-        // Gaussian elimination on the 4x4 SPD system (diagonal pivots).
+        // Gaussian elimination on the 3x3 SPD system (diagonal pivots).
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 3; ++k) {
 #pragma unroll
-          for (int i = k + 1; i < 4; ++i) {
+          for (int i = k + 1; i < 3; ++i) {
             double f = J[i][k] / J[k][k];
 #pragma unroll
-            for (int j = k; j < 4; ++j) J[i][j] -= f * J[k][j];
+            for (int j = k; j < 3; ++j) J[i][j] -= f * J[k][j];
             rr[i] -= f * rr[k];
           }
         }
-        double d[4];
+        double d[3];
 #pragma unroll
-        for (int k = 3; k >= 0; --k) {
+        for (int k = 2; k >= 0; --k) {
           double acc = rr[k];
 #pragma unroll
-          for (int j = k + 1; j < 4; ++j) acc -= J[k][j] * d[j];
+          for (int j = k + 1; j < 3; ++j) acc -= J[k][j] * d[j];
           d[k] = acc / J[k][k];
         }
+        double step = 0.0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) lam[k] -= d[k];
+        for (int s = 0; s < NOUT; ++s) {
+          double e = fabs(A[0][s] * d[0] + A[1][s] * d[1] + A[2][s] * d[2]);
+          step = e > step ? e : step;
+        }
+        const double damp = step > 2.0 ? 2.0 / step : 1.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lam[k] -= damp * d[k];
+        if (!(step > 1e-14)) break;   // converged (or NaN: caught by the checks below)
       }
-      double Yf[NOUT];
-      res = 0.0;
+      double Yf[NOUT], zf = 0.0;
 #pragma unroll
       for (int s = 0; s < NOUT; ++s) {
-        double e = A[0][s] * lam[0] + A[1][s] * lam[1] + A[2][s] * lam[2] + lam[3];
-        Yf[s] = Yp[s] * exp(e);
+        Yf[s] = Yp[s] * exp(A[0][s] * lam[0] + A[1][s] * lam[1] + A[2][s] * lam[2]);
+        zf += Yf[s];
       }
+      res = 0.0;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < 3; ++i) {
         double acc = -cvec[i], scale = 0.0;
 #pragma unroll
         for (int s = 0; s < NOUT; ++s) {
@@ -386,6 +402,8 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
         double rel = fabs(acc) / (scale + 1e-300);
         res = rel > res ? rel : res;
       }
+#pragma unroll
+      for (int s = 0; s < NOUT; ++s) Yf[s] /= zf;
       // Out-of-hull check on the raw features.
       bool inside = true;
       {
@@ -399,7 +417,9 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
         f[13] = log(h);
 #pragma unroll
         for (int k = 0; k < NIN; ++k) {
-          double span = cst->inv_std[k] > 0.f ? cst->hull_margin / cst->inv_std[k] : 0.0;
+          // margin in standard deviations, plus the FP32 rounding of the bounds
+          double span = (cst->inv_std[k] > 0.f ? cst->hull_margin / cst->inv_std[k] : 0.0) +
+                        1e-6 * (1.0 + fabs(f[k]));
           inside = inside && f[k] >= cst->hull_lo[k] - span && f[k] <= cst->hull_hi[k] + span;
         }
       }

@@ -92,6 +92,18 @@ def lib():
         L.fsg_chem_react.argtypes = [vp, vp, i64, i64, vp, f64, ctypes.POINTER(ChemParams), vp]
         L.fsg_chem_react.restype = ctypes.c_int
         L.fsg_chem_last_error.restype = ctypes.c_char_p
+        f32 = ctypes.c_float
+        L.fsg_mlp_linear_forward.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]
+        L.fsg_mlp_linear_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]
+        L.fsg_mlp_mse_grad.argtypes = [vp, vp, vp, i64, f32, vp]
+        L.fsg_mlp_adam.argtypes = [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]
+        L.fsg_mlp_infer.argtypes = [vp, vp, vp, vp, vp, vp, f64, vp, vp, i64, i32, vp]
+        for name in ("fsg_mlp_linear_forward", "fsg_mlp_linear_backward", "fsg_mlp_mse_grad",
+                     "fsg_mlp_adam", "fsg_mlp_infer"):
+            getattr(L, name).restype = ctypes.c_int
+        L.fsg_mlp_packed_weight_count.restype = ctypes.c_int32
+        L.fsg_mlp_consts_size.restype = ctypes.c_int32
+        L.fsg_mlp_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -105,6 +117,12 @@ def check_chem(rc):
     if rc != 0:
         raise RuntimeError(f"libfluid_gfx950 chemistry error {rc}: "
                            f"{lib().fsg_chem_last_error().decode()}")
+
+
+def check_mlp(rc):
+    if rc != 0:
+        raise RuntimeError(f"libfluid_gfx950 surrogate error {rc}: "
+                           f"{lib().fsg_mlp_last_error().decode()}")
 
 
 def ptr(t):

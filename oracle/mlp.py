@@ -6,10 +6,13 @@ Network: x (14) -> [W1 x + b1, SiLU] -> 128 -> [SiLU] -> 128 -> [SiLU] -> 128
 -> W4 h + b4 (11).  Loss: mean squared error over a batch and the 11
 outputs.  Adam: the standard update (Kingma & Ba 2015) with bias correction.
 
-Projection: given a positive prediction Y and the element/charge/mass
-constraints A Y = c (rows: N atoms per kg, O atoms per kg, charge per kg,
-total mass fraction), the minimum-KL point Y' = Y exp(A^T lam) / normaliser
-solves the constraints; lam comes from Newton on A (Y exp(A^T lam)) = c.
+Projection: given a positive prediction Y and the element/charge
+constraints A Y = c (rows: N atoms per kg, O atoms per kg, charge per kg),
+the minimum-KL point Y' = Y exp(A^T lam) solves them; lam comes from a damped
+Newton iteration on A (Y exp(A^T lam)) = c.  The total mass is restored
+afterwards by normalising Y' (the integrator's own final rescale): with the
+species table giving ions the neutral molar mass, sum(Y) equals
+M_N (N row) + M_O (O row) + Y_e, so a mass row would freeze Y_e.
 """
 
 import numpy as np
@@ -22,9 +25,8 @@ O_ATOMS = np.array([0, 0, 1, 1, 0, 2, 0, 1, 1, 0, 2], float)
 
 
 def constraint_matrix():
-    """Rows per unit mass: mol N, mol O, mol charge, mass fraction sum."""
-    return np.stack([N_ATOMS / MOLAR_MASS, O_ATOMS / MOLAR_MASS, CHARGE / MOLAR_MASS,
-                     np.ones(11)])
+    """Rows per unit mass: mol N, mol O, mol charge."""
+    return np.stack([N_ATOMS / MOLAR_MASS, O_ATOMS / MOLAR_MASS, CHARGE / MOLAR_MASS])
 
 
 def silu(z):
@@ -77,13 +79,20 @@ def adam_step(params, grads, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     return out
 
 
-def project(Y, c, iters=8):
-    """Minimum-KL projection of positive Y onto A Y' = c (Newton on lam)."""
+def project(Y, c, iters=32):
+    """Minimum-KL projection of positive Y onto A Y' = c: Newton on the dual
+    variable lam, each step scaled so that no species' log changes by more
+    than 2 (the electron column of A is ~1e6 times the others), stopping
+    once the step is below 1e-14."""
     A = constraint_matrix()
-    lam = np.zeros(4)
+    lam = np.zeros(3)
     for _ in range(iters):
         Yp = Y * np.exp(A.T @ lam)
         r = A @ Yp - c
         J = (A * Yp) @ A.T
-        lam = lam - np.linalg.solve(J, r)
+        d = np.linalg.solve(J, r)
+        step = np.max(np.abs(A.T @ d))
+        lam = lam - (2.0 / step if step > 2.0 else 1.0) * d
+        if not step > 1e-14:
+            break
     return Y * np.exp(A.T @ lam)
