@@ -170,6 +170,116 @@ void fsg_destroy(fsg_context* ctx);
 const char* fsg_last_error(void);
 int32_t fsg_abi_version(void);
 
+/* ---- MOOD: troubled-cell detection, subcell FV fallback, positivity --------
+ * HybridMOODSubcellLimiter (hybrid/limiter.py:267-343) with its
+ * HybridTroubledCellSensor (:150-185), HybridPositivityLimiter (:11-147) and
+ * HybridSubcellFV (hybrid/subcell.py:442-950), for affine elements.  A MOOD
+ * context extends a residual context (same blocks, face groups, dtype). */
+typedef struct fsg_mood fsg_mood;
+
+/* Element-independent tables of one element type (all reals in the context
+ * dtype; built by hybrid/mood.py). */
+typedef struct {
+  int32_t N;   /* nodes */
+  int32_t Q;   /* volume points */
+  int32_t NS;  /* sample points: the Q volume points, then the face sample points */
+  int32_t NI;  /* internal subcell faces */
+  int32_t NB;  /* boundary subfaces */
+  int32_t reserved;
+  const void* p1proj;       /* (N, N) P1 projector at the nodes (sensor) */
+  const void* samples;      /* (NS, N) basis at the sample points */
+  const void* wq;           /* (Q) reference volume weights */
+  const void* P;            /* (64, N) subcell averages of the basis */
+  const void* R;            /* (N, 64) least-squares reconstruction */
+  const void* subvol;       /* (64) reference subcell volumes */
+  const int32_t* int_o;     /* (NI) owner subcell */
+  const int32_t* int_n;     /* (NI) neighbour subcell */
+  const void* int_area;     /* (NI, 3) reference area vector, owner -> neighbour */
+  const void* int_dc;       /* (NI, 3) reference centre offset neighbour - owner */
+  const int32_t* int_ptr;   /* (65) per-subcell list of internal faces */
+  const int32_t* int_list;  /* (2 NI) */
+  const int32_t* bnd_face;  /* (NB) parent local face of each boundary subface */
+  const int32_t* bnd_gather;/* (NB) subcell read as the subface's FV trace */
+  const int32_t* tgt_ptr;   /* (65) per-subcell list of boundary subfaces it receives */
+  const int32_t* tgt_list;
+} fsg_mood_type;
+
+/* Per element block: each element's face entries, max_faces pairs of int32
+ * (group | local_face << 16 | side << 24, face index); group -1 = unused. */
+typedef struct {
+  int32_t max_faces;
+  int32_t reserved;
+  const int32_t* elem_faces; /* (E, max_faces, 2) */
+  int32_t* flags;            /* (E) troubled flags, written by the MOOD calls */
+  void* sensor;              /* (E) sensor values written by detection (may be null) */
+} fsg_mood_block;
+
+/* Per face group: K subface pairs per face (hybrid/subcell.py:473-533),
+ * de-duplicated into tables: pair_ids (tables, K, 2) = owner / neighbour
+ * boundary-subface ids, pair_geom (tables, K, 4) = side giving the normal
+ * (0 owner, 1 neighbour) and its reference area vector oriented owner ->
+ * neighbour, basis_o / basis_n (tables, K, N) = basis at the subface
+ * centres on each side. */
+typedef struct {
+  int32_t K;
+  int32_t reserved;
+  const int32_t* pair_table; /* (F) */
+  const int32_t* pair_ids;
+  const void* pair_geom;
+  const void* basis_o;
+  const void* basis_n;       /* null on boundary groups */
+} fsg_mood_group;
+
+typedef struct {
+  double threshold;        /* sensor threshold (hybrid/limiter.py:150) */
+  double density_floor;    /* positivity floors (hybrid/limiter.py:11-26) */
+  double pressure_floor;
+  double relative_floor;
+  int32_t reconstruction;  /* 0 constant, 1 tvd, 2 weno (hybrid/subcell.py:541-665) */
+  int32_t sensor_on;       /* degree >= 2 */
+} fsg_mood_params;
+
+/* types[t] for t = 0 hexahedron, 1 wedge, 2 tetra (entries of absent types
+ * are ignored); blocks / groups in the order of the residual context.
+ * counter (1 int32) and list (sum of block sizes int32) are device scratch. */
+int fsg_mood_create(fsg_context* ctx, const fsg_mood_params* params, const fsg_mood_type* types,
+                    const fsg_mood_block* blocks, const fsg_mood_group* groups, int32_t* counter,
+                    int32_t* list, fsg_mood** out);
+void fsg_mood_destroy(fsg_mood* m);
+
+/* HybridMOODSubcellLimiter.troubled (hybrid/limiter.py:285-300): sensor or
+ * PAD failure of U, into the blocks' flags. */
+int fsg_mood_troubled(fsg_mood* m, const void* const* U, void* stream);
+
+/* HybridPositivityLimiter.__call__ (hybrid/limiter.py:141-147): out may
+ * alias U. */
+int fsg_mood_positivity(fsg_mood* m, const void* const* U, void* const* out, void* stream);
+
+/* HybridMOODSubcellLimiter.recompute (hybrid/limiter.py:302-338): detection
+ * on `detect` (or on dg_candidate when null) unless use_flags != 0 (then
+ * the blocks' flags are taken as given), the subcell FV recompute of the
+ * troubled elements from `states` with the neighbours' mean correction, then
+ * positivity.  visc (may be null) is the additional rhs; out may alias
+ * dg_candidate. */
+int fsg_mood_recompute(fsg_mood* m, const void* const* states, const void* const* dg_candidate,
+                       const void* const* detect, const void* const* visc, double dt,
+                       const void* const* dt_elem, int32_t use_flags,
+                       const fsg_boundary* boundary, void* const* out, void* stream);
+
+/* The MOOD branch of HybridSSPRK3.rk_stage (hybrid/time_integration.py:
+ * 404-439) after the residuals: euler = curr + dt (inv + visc), detection on
+ * alpha base + beta euler, recompute, positivity, blend with base,
+ * positivity, admissibility into diag.  work holds euler (may alias next). */
+int fsg_mood_stage(fsg_mood* m, const void* const* base, const void* const* curr,
+                   const void* const* inv, const void* const* visc, void* const* work,
+                   void* const* next, double dt, const void* const* dt_elem, double alpha,
+                   double beta, const fsg_boundary* boundary, fsg_stage_diag* diag,
+                   void* stream);
+
+/* Number of troubled elements found by the last detection (device counter,
+ * read synchronously on the stream). */
+int fsg_mood_count(fsg_mood* m, int64_t* count, void* stream);
+
 /* ---- Park 11-species air chemistry (FP64) ---------------------------------
  * table: device buffer of fsg_chem_table_size() doubles holding the
  * mechanism (molar masses, NASA-7 fits, Arrhenius and temperature modes,

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/fluid_gfx950.h"
+#include "fsg_context.h"
 #include "fsg_dispatch.h"
 
 namespace fsg {
@@ -30,7 +31,7 @@ __global__ void diag_reset_kernel(StageDiag* d) {
 
 static thread_local std::string g_error;
 
-static int fail(int code, const char* fmt, ...) {
+int fsg_fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -39,20 +40,9 @@ static int fail(int code, const char* fmt, ...) {
   g_error = buf;
   return code;
 }
+#define fail fsg_fail
 
-struct fsg_context {
-  fsg_params params;
-  std::vector<fsg_block_desc> blocks;
-  std::vector<fsg_group_desc> groups;
-  const int32_t* support;
-  void* face_out;
-  // Optional per-launch timing: an event pair around every launch, on the
-  // launch stream; kind = group index, or ngroups + block index.
-  bool timing = false;
-  std::vector<hipEvent_t> pool;
-  std::vector<int> kinds;
-  size_t used = 0;
-};
+
 
 namespace {
 // Records the start event of a timed launch; returns the pair index or -1.

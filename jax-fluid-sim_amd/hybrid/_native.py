@@ -42,6 +42,31 @@ class GroupDesc(ctypes.Structure):
         ("out_owner", ctypes.c_int64), ("out_neighbor", ctypes.c_int64)]
 
 
+class MoodType(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("N", "Q", "NS", "NI", "NB", "reserved")] + [
+        (k, ctypes.c_void_p) for k in ("p1proj", "samples", "wq", "P", "R", "subvol", "int_o",
+                                       "int_n", "int_area", "int_dc", "int_ptr", "int_list",
+                                       "bnd_face", "bnd_gather", "tgt_ptr", "tgt_list")]
+
+
+class MoodBlock(ctypes.Structure):
+    _fields_ = [("max_faces", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("elem_faces", ctypes.c_void_p), ("flags", ctypes.c_void_p),
+                ("sensor", ctypes.c_void_p)]
+
+
+class MoodGroup(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+        (k, ctypes.c_void_p) for k in ("pair_table", "pair_ids", "pair_geom", "basis_o",
+                                       "basis_n")]
+
+
+class MoodParams(ctypes.Structure):
+    _fields_ = [("threshold", ctypes.c_double), ("density_floor", ctypes.c_double),
+                ("pressure_floor", ctypes.c_double), ("relative_floor", ctypes.c_double),
+                ("reconstruction", ctypes.c_int32), ("sensor_on", ctypes.c_int32)]
+
+
 class ChemParams(ctypes.Structure):
     _fields_ = [("newton_iters", ctypes.c_int32), ("dimensionless", ctypes.c_int32),
                 ("gamma", ctypes.c_double), ("T_ref", ctypes.c_double),
@@ -92,6 +117,20 @@ def lib():
         L.fsg_chem_react.argtypes = [vp, vp, i64, i64, vp, f64, ctypes.POINTER(ChemParams), vp]
         L.fsg_chem_react.restype = ctypes.c_int
         L.fsg_chem_last_error.restype = ctypes.c_char_p
+        pp = ctypes.POINTER(vp)
+        L.fsg_mood_create.argtypes = [vp, ctypes.POINTER(MoodParams), ctypes.POINTER(MoodType),
+                                      ctypes.POINTER(MoodBlock), ctypes.POINTER(MoodGroup), vp, vp,
+                                      ctypes.POINTER(vp)]
+        L.fsg_mood_destroy.argtypes = [vp]
+        L.fsg_mood_destroy.restype = None
+        L.fsg_mood_troubled.argtypes = [vp, pp, vp]
+        L.fsg_mood_positivity.argtypes = [vp, pp, pp, vp]
+        L.fsg_mood_recompute.argtypes = [vp, pp, pp, pp, pp, f64, pp, i32, vp, pp, vp]
+        L.fsg_mood_stage.argtypes = [vp, pp, pp, pp, pp, pp, pp, f64, pp, f64, f64, vp, vp, vp]
+        L.fsg_mood_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), vp]
+        for name in ("fsg_mood_create", "fsg_mood_troubled", "fsg_mood_positivity",
+                     "fsg_mood_recompute", "fsg_mood_stage", "fsg_mood_count"):
+            getattr(L, name).restype = ctypes.c_int
         f32 = ctypes.c_float
         L.fsg_mlp_linear_forward.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]
         L.fsg_mlp_linear_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]

@@ -110,18 +110,48 @@ class HybridSSPRK3:
                                          float(beta), op.rhs_terms(), ctypes.byref(b),
                                          nat.ptr(self._diag.buf), stream))
 
+    def _mood_stage(self, base, curr, nxt, dt, alpha, beta, boundary_state):
+        """The MOOD branch of hybrid/time_integration.py:404-439: inviscid and
+        viscous rhs, then the fused detection / subcell recompute / positivity
+        / blend / positivity / admissibility sequence."""
+        op = self.operator
+        stream = nat.stream_handle(self.solver.device)
+        self._diag.reset(stream)
+        if hasattr(op, "inviscid_rhs"):
+            inv = op.inviscid_rhs(curr, boundary_state)
+            # a zero viscosity gives an exactly zero viscous rhs: skip it
+            visc = (op.viscous_rhs(curr, boundary_state) if self.solver.dynamic_viscosity > 0.0
+                    else None)
+        else:
+            inv, visc = op.rhs(curr, boundary_state), None
+        work = {t: torch.empty_like(curr[t]) for t in op.block_types}
+        self.limiter.stage(base, curr, inv, visc, work, nxt, dt, alpha, beta, boundary_state,
+                           nat.ptr(self._diag.buf))
+
     def rk_stage(self, base_states, current_states, dt, base_weight, current_weight,
                  boundary_state=None):
-        """hybrid/time_integration.py:404-439 (limiter=None)."""
+        """hybrid/time_integration.py:404-439."""
         nxt = {t: torch.empty_like(current_states[t]) for t in self.operator.block_types}
-        self._stage(base_states, current_states, nxt, dt, base_weight, current_weight,
-                    boundary_state, True)
+        if self.limiter is not None:
+            self._mood_stage(base_states, current_states, nxt, dt, base_weight, current_weight,
+                             boundary_state)
+        else:
+            self._stage(base_states, current_states, nxt, dt, base_weight, current_weight,
+                        boundary_state, True)
         return (nxt, *self._diag.values(self.density_floor, self.pressure_floor))
 
     def checked_step(self, states, dt, boundary_state=None):
-        """hybrid/time_integration.py:441-583 (non-MOOD branch): three fused
-        stages; valid = all stages valid, minima over the stages."""
+        """hybrid/time_integration.py:441-583.  Without a limiter: three fused
+        stages, valid = all stages valid, minima over the stages.  With MOOD
+        (:463-554): three MOOD stages, diagnostics of the last one."""
         types = self.operator.block_types
+        if self.limiter is not None:
+            s = states
+            for a, b in RK3_WEIGHTS:
+                nxt = {t: torch.empty_like(states[t]) for t in types}
+                self._mood_stage(states, s, nxt, dt, a, b, boundary_state)
+                s = nxt
+            return (s, *self._diag.values(self.density_floor, self.pressure_floor))
         if self._buffers is None or any(self._buffers[0][t].shape != states[t].shape
                                         for t in types):
             self._buffers = [{t: torch.empty_like(states[t]) for t in types} for _ in range(2)]
