@@ -98,6 +98,90 @@ def run_path(topo, dtype, steps, warmup, dt):
                 face_ms=gms[0] / max(gn[0], 1), face_launches=gn[0])
 
 
+def shock_bubble_primitive(x):
+    """Mach 2.5 planar shock at x = 0.1 moving into rho = 1, p = 1, u = 0
+    (Rankine-Hugoniot post-shock state behind it) and a Gaussian density
+    bubble rho = 1 + 4 exp(-|x - (0.4, 0.5, 0.5)|^2 / 0.15^2) in pressure
+    equilibrium."""
+    M, g = 2.5, GAMMA
+    r2 = (g + 1) * M * M / ((g - 1) * M * M + 2)
+    p2 = 1 + 2 * g / (g + 1) * (M * M - 1)
+    u2 = M * np.sqrt(g) * (1 - 1 / r2)
+    post = x[..., 0] < 0.1
+    d2 = ((x - np.array([0.4, 0.5, 0.5])) ** 2).sum(-1)
+    rho = np.where(post, r2, 1.0 + 4.0 * np.exp(-d2 / 0.15 ** 2))
+    zero = np.zeros_like(rho)
+    return np.stack([rho, np.where(post, u2, 0.0), zero, zero, np.where(post, p2, 1.0)], -1), \
+        np.array([r2, u2, 0.0, 0.0, p2])
+
+
+def mood_rate(n=64, split=0.2, steps=5, seed=1):
+    """Shock-bubble on the unit box of n^3 hexahedra with a seeded fraction
+    split into wedges / tetrahedra, p=2, HLLC + BR2 (Re 1e4 on the post-shock
+    speed), MOOD (sensor 0.05, WENO, 64 subcells, reference subface
+    indexing), positivity floors 1e-7 / relative 0.05; inflow = post-shock
+    farfield, outflow transmissive, slip walls.  Same dt for FP64 and FP32."""
+    import torch
+    from hybrid import (FarfieldWallBoundary, HybridCompressibleSolver, HybridMOODSubcellLimiter,
+                        HybridNavierStokesOperator, HybridPositivityLimiter, HybridSSPRK3,
+                        box_mesh)
+    names = {"xmin": "inlet", "xmax": "outlet", "ymin": "walls", "ymax": "walls",
+             "zmin": "walls", "zmax": "walls"}
+    mesh = box_mesh(n, split_fraction=split, seed=seed, plane_names=names)
+    out = {"workload": f"shock-bubble, unit box {n}^3 hexahedra, {int(split * 100)}% split "
+                       "(seeded, wedges/tetrahedra), p=2, MOOD + positivity, Mach 2.5"}
+    res, dt = {}, None
+    for key, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        t0 = time.perf_counter()
+        _, far = shock_bubble_primitive(np.zeros((1, 3)))
+        solver = HybridCompressibleSolver(mesh, polynomial_degree=2, gamma=GAMMA,
+                                          dynamic_viscosity=far[1] / 1.0e4, prandtl=PR, dtype=dtype)
+        op = HybridNavierStokesOperator(solver)
+        pos = HybridPositivityLimiter(op, density_floor=1e-7, pressure_floor=1e-7)
+        lim = HybridMOODSubcellLimiter(op, threshold=0.05, positivity=pos, reconstruction="weno")
+        rk = HybridSSPRK3(op, cfl=0.3, limiter=lim)
+        bnd = FarfieldWallBoundary(solver, far)
+        U0 = solver.state_blocks_from_primitive_function(lambda x: shock_bubble_primitive(x)[0])
+        if dt is None:
+            dt = float(rk.stable_dt(U0))
+        U, valid, _, _ = rk.checked_step(U0, dt, bnd)        # warm-up (builds the MOOD tables)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        U = U0
+        (U, valid, rmin, pmin), ms = _timed(lambda: _steps(rk, U0, dt, bnd, steps))
+        E = sum(b.count for b in solver.mesh.blocks)
+        dof = sum(v.shape[0] * v.shape[1] for v in U.values())
+        res[key] = dict(state=U, ms_per_step=ms / steps,
+                        dof_updates_per_s_per_stage=dof * 3 * steps / (ms * 1e-3),
+                        troubled_last_stage=lim.troubled_count(), elements=E, valid=bool(valid),
+                        setup_s=setup)
+        out["elements"] = {b.element_type: b.count for b in solver.mesh.blocks}
+        out["dof_per_var"] = dof
+        print(f"[bench] MOOD {key}: {ms / steps:.3f} ms/step, troubled "
+              f"{res[key]['troubled_last_stage']} / {E}, valid {bool(valid)}", file=sys.stderr,
+              flush=True)
+        del rk, lim, pos, op, solver
+        torch.cuda.empty_cache()
+    for key in ("f64", "f32"):
+        r = res[key]
+        out[key] = {k: r[k] for k in ("ms_per_step", "dof_updates_per_s_per_stage",
+                                      "troubled_last_stage", "valid", "setup_s")}
+    a, b = res["f32"]["state"], res["f64"]["state"]
+    A = torch.cat([a[k].double().reshape(-1, 5) for k in b])
+    B = torch.cat([b[k].reshape(-1, 5) for k in b])
+    err = (torch.linalg.vector_norm(A - B, dim=0) / torch.linalg.vector_norm(B, dim=0)).cpu()
+    out["fp32_rel_l2_vs_fp64"] = dict(zip(["rho", "rho_u", "rho_v", "rho_w", "rho_E"],
+                                          map(float, err)))
+    out["steps"], out["dt"] = steps, dt
+    return out
+
+
+def _steps(rk, U, dt, bnd, steps):
+    for _ in range(steps):
+        U, valid, rmin, pmin = rk.checked_step(U, dt, bnd)
+    return U, valid, rmin, pmin
+
+
 def _timed(fn):
     import torch
     torch.cuda.synchronize()
@@ -220,6 +304,7 @@ def main():
     ap.add_argument("--n", type=int, default=32, help="elements per box edge")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-chem", action="store_true")
+    ap.add_argument("--skip-mood", action="store_true")
     ap.add_argument("--only", choices=["f64", "f32"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -283,6 +368,8 @@ def main():
     }
     if not args.skip_chem:
         out["chemistry"] = chemistry_rate(1 << 20)
+    if not args.skip_mood:
+        out["mood"] = mood_rate()
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

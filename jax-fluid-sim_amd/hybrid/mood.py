@@ -47,12 +47,14 @@ class MoodTables:
             shapes, _ = face_quadrature(("quad", "triangle")[g["face_type"]], p)
             nv = shapes.shape[1]
             sel = g["sel"]
-            to = topo.blocks[g["owner_block"]].element_type
-            pts[to].append(np.einsum("qc,fcr->fqr", shapes, f.owner_corners[sel, :nv]).reshape(-1, 3))
+            sides = [(g["owner_block"], f.owner_corners)]
             if g["neighbor_block"] >= 0:
-                tn = topo.blocks[g["neighbor_block"]].element_type
-                pts[tn].append(np.einsum("qc,fcr->fqr", shapes,
-                                         f.neighbor_corners[sel, :nv]).reshape(-1, 3))
+                sides.append((g["neighbor_block"], f.neighbor_corners))
+            for blk, corners in sides:
+                # the points depend only on the face's reference corners
+                c = np.unique(np.round(corners[sel, :nv].reshape(len(sel), -1), 12), axis=0)
+                pts[topo.blocks[blk].element_type].append(
+                    np.einsum("qc,fcr->fqr", shapes, c.reshape(-1, nv, 3)).reshape(-1, 3))
         self.types = (nat.MoodType * 3)()
         for t in types_present:
             R = reference_tables(t, p)
@@ -101,8 +103,8 @@ class MoodTables:
             idx = np.arange(F)
             ob, nb = g["owner_block"], g["neighbor_block"]
             oe = f.owner_local[sel]
-            # elements appear at most once per group and side, so the
-            # vectorised fill below never collides within one call
+            # an element can own several faces of one group: fill in rounds,
+            # each round touching every element at most once
             for side, blk, elems, lfs in ((0, ob, oe, f.owner_face[sel]),) + (
                     ((1, nb, f.neighbor_local[sel], f.neighbor_face[sel]),) if nb >= 0 else ()):
                 code = gi | (lfs.astype(np.int64) << 16) | (side << 24)
@@ -155,11 +157,13 @@ class MoodTables:
         ob, nb = g["owner_block"], g["neighbor_block"]
         to = topo.blocks[ob].element_type
         tn = topo.blocks[nb].element_type if nb >= 0 else None
+        nv = 4 if g["face_type"] == 0 else 3          # triangles leave corner 4 unset
         keys = np.concatenate([f.owner_face[sel, None].astype(float),
-                               np.round(f.owner_corners[sel].reshape(len(sel), -1), 9)], 1)
+                               np.round(f.owner_corners[sel, :nv].reshape(len(sel), -1), 9)], 1)
         if nb >= 0:
             keys = np.concatenate([keys, f.neighbor_face[sel, None].astype(float),
-                                   np.round(f.neighbor_corners[sel].reshape(len(sel), -1), 9)], 1)
+                                   np.round(f.neighbor_corners[sel, :nv].reshape(len(sel), -1),
+                                            9)], 1)
         uniq, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
         ids, geom, bo, bn = [], [], [], []
         K = None

@@ -202,3 +202,29 @@ def test_checked_step_with_mood():
     got, valid, _, _ = HybridSSPRK3(op, limiter=lim).checked_step(_dev(solver, states), dt, b)
     assert bool(valid) == bool(wvalid)
     assert np.all(rel_l2(_np(got), want) <= 1e-11)
+
+
+FP32_TOL = 1e-4
+
+
+@pytest.mark.parametrize("name,p", [("split", 2), ("split_periodic", 2)])
+def test_fp32_path(name, p):
+    """FP32 MOOD kernels: identical troubled sets (every sensor value is at
+    least 4e-4 away from the threshold on these states, far above FP32
+    rounding of the ratio) and the stage within 1e-4 relative L2."""
+    solver, op, b, lim, oop, ob, olim, states = _case(name, p, dtype=torch.float32)
+    sensor, _ = olim.sensor(states)
+    for t, s in sensor.items():
+        assert np.all(np.abs(s - olim.sensor.threshold) > 1e-4), t
+    want = olim.troubled(states)
+    got = lim.troubled(_dev(solver, states))
+    for t in want:
+        assert np.array_equal(got[t].cpu().numpy(), want[t]), t
+    curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x, seed=9))
+            for t, x in oop.geometry.solution_node_coordinates.items()}
+    dt = 2e-3
+    w, wv, _, _ = SSPRK3(oop, limiter=olim).rk_stage(states, curr, dt, 0.75, 0.25, ob)
+    g, gv, _, _ = HybridSSPRK3(op, limiter=lim).rk_stage(_dev(solver, states), _dev(solver, curr),
+                                                         dt, 0.75, 0.25, b)
+    assert bool(gv) == bool(wv)
+    assert np.all(rel_l2(_np(g), w) <= FP32_TOL)
