@@ -305,12 +305,21 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-chem", action="store_true")
     ap.add_argument("--skip-mood", action="store_true")
-    ap.add_argument("--only", choices=["f64", "f32"], default=None)
+    ap.add_argument("--only", choices=["f64", "f32", "mood", "chem"], default=None,
+                    help="run one path of the residual benchmark, or only the MOOD or "
+                         "chemistry section (for profiling; prints that section's JSON)")
+    ap.add_argument("--mood-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo exchange is a design, not a build)", file=sys.stderr)
         sys.exit(2)
+    if args.only == "mood":
+        print(json.dumps(mood_rate(steps=args.mood_steps)))
+        return
+    if args.only == "chem":
+        print(json.dumps(chemistry_rate(1 << 20)))
+        return
 
     import torch
     from hybrid import Topology, box_mesh
@@ -369,7 +378,7 @@ def main():
     if not args.skip_chem:
         out["chemistry"] = chemistry_rate(1 << 20)
     if not args.skip_mood:
-        out["mood"] = mood_rate()
+        out["mood"] = mood_rate(steps=args.mood_steps)
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

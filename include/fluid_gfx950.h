@@ -75,6 +75,7 @@ typedef struct {
   const int64_t* rec_off; /* (R) offsets in reals into face_out */
   const int32_t* rec_sup; /* (R) offsets into the support-index array */
   const int32_t* rec_nf;  /* (R) support size */
+  const void* minvT; /* (N, N) transposed reference inverse mass */
 } fsg_block_desc;
 
 /* One face group: (face type, owner block, neighbour block) as in
@@ -166,8 +167,20 @@ int fsg_timing(fsg_context* ctx, int32_t enable);
 int fsg_timing_read(fsg_context* ctx, double* group_ms, double* block_ms,
                     int64_t* group_launches, int64_t* block_launches);
 
+/* Context options (no counterpart in the reference).
+ * FSG_OPT_ELEMENT_MFMA: 1 (default) runs the element operators as batched
+ * matrix-core products, 0 as one workgroup per element on the vector units;
+ * both compute the same discrete operator. */
+#define FSG_OPT_ELEMENT_MFMA 1
+int fsg_set_option(fsg_context* ctx, int32_t option, int32_t value);
+
 void fsg_destroy(fsg_context* ctx);
 const char* fsg_last_error(void);
+
+/* C = A B for 16x16 row-major matrices through the 16x16x4 matrix-core tiles
+ * the element kernel uses (no counterpart in the reference: a device check
+ * of the operand / accumulator lane layout). */
+int fsg_mfma_selftest(int32_t dtype, const void* A, const void* B, void* C, void* stream);
 int32_t fsg_abi_version(void);
 
 /* ---- MOOD: troubled-cell detection, subcell FV fallback, positivity --------
@@ -186,11 +199,12 @@ typedef struct {
   int32_t NI;  /* internal subcell faces */
   int32_t NB;  /* boundary subfaces */
   int32_t reserved;
-  const void* p1proj;       /* (N, N) P1 projector at the nodes (sensor) */
-  const void* samples;      /* (NS, N) basis at the sample points */
+  const void* p1proj;       /* (N, N) P1 projector at the nodes, transposed (sensor) */
+  const void* samples;      /* (N, NS) basis at the sample points, point fastest */
   const void* wq;           /* (Q) reference volume weights */
   const void* P;            /* (64, N) subcell averages of the basis */
-  const void* R;            /* (N, 64) least-squares reconstruction */
+  const void* PT;           /* (N, 64) the same, subcell fastest */
+  const void* RT;           /* (64, N) least-squares reconstruction R (N, 64), transposed */
   const void* subvol;       /* (64) reference subcell volumes */
   const int32_t* int_o;     /* (NI) owner subcell */
   const int32_t* int_n;     /* (NI) neighbour subcell */

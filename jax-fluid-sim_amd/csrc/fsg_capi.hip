@@ -13,7 +13,7 @@
 
 namespace fsg {
 template <int P, int V, typename real>
-int launch_element(int, int64_t, const ElemArgs<real>&, const Phys<real>&, hipStream_t);
+int launch_element(int, int64_t, const ElemArgs<real>&, const Phys<real>&, hipStream_t, int);
 template <int P, int V, typename real>
 int launch_face(int, int, int, int, int64_t, const FaceArgs<real>&, const Phys<real>&, hipStream_t);
 }  // namespace fsg
@@ -148,6 +148,7 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.D = static_cast<const real*>(d.D);
     a.wq = static_cast<const real*>(d.wq);
     a.minv = static_cast<const real*>(d.minv);
+    a.minvT = static_cast<const real*>(d.minvT);
     a.jinv = static_cast<const real*>(d.jinv);
     a.scale = static_cast<const real*>(d.scale);
     a.detw = static_cast<const real*>(d.detw);
@@ -165,7 +166,7 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.terms = terms;
     a.diag = reinterpret_cast<StageDiag*>(diag);
     long ev = timing_begin(c, int(c->groups.size() + b), s);
-    int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s);
+    int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s, c->element_mfma);
     timing_end(c, ev, s);
     if (rc) return fail(rc, "element launch failed (type %d): %s", d.element_type,
                         hipGetErrorString(hipGetLastError()));
@@ -260,6 +261,15 @@ int fsg_diag_reset(fsg_stage_diag* diag, void* stream) {
                      static_cast<hipStream_t>(stream), reinterpret_cast<fsg::StageDiag*>(diag));
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : fail(-1, "diag reset: %s", hipGetErrorString(e));
+}
+
+int fsg_set_option(fsg_context* ctx, int32_t option, int32_t value) {
+  if (!ctx) return fail(-10, "null argument");
+  if (option == FSG_OPT_ELEMENT_MFMA) {
+    ctx->element_mfma = value != 0;
+    return 0;
+  }
+  return fail(-13, "unknown option %d", option);
 }
 
 int fsg_timing(fsg_context* ctx, int32_t enable) {

@@ -2,6 +2,7 @@
 // precision) to the compile-time kernel instantiations.
 #pragma once
 #include "fsg_kernels.h"
+#include "fsg_elem_mfma.h"
 
 namespace fsg {
 
@@ -9,7 +10,7 @@ constexpr int ELEM_THREADS = 128;
 
 template <int P, int V, typename real>
 int launch_element(int type, int64_t count, const ElemArgs<real>& a, const Phys<real>& ph,
-                   hipStream_t s);
+                   hipStream_t s, int mfma);
 
 template <int P, int V, typename real>
 int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64_t count,
@@ -18,8 +19,17 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
 #define FSG_DEFINE_LAUNCHERS(P, V, real)                                                        \
   template <>                                                                                   \
   int launch_element<P, V, real>(int type, int64_t count, const ElemArgs<real>& a,              \
-                                 const Phys<real>& ph, hipStream_t s) {                        \
+                                 const Phys<real>& ph, hipStream_t s, int mfma) {              \
     if (count <= 0) return 0;                                                                   \
+    if (mfma) {                                                                                 \
+      switch (type) {                                                                           \
+        FSG_MFMA_CASE(HEX, P, V, real)                                                          \
+        FSG_MFMA_CASE(WEDGE, P, V, real)                                                        \
+        FSG_MFMA_CASE(TET, P, V, real)                                                          \
+        default: return -2;                                                                     \
+      }                                                                                         \
+      return hipGetLastError() == hipSuccess ? 0 : -1;                                          \
+    }                                                                                           \
     dim3 grid((unsigned)count), block(ELEM_THREADS);                                            \
     switch (type) {                                                                             \
       case HEX: hipLaunchKernelGGL((element_kernel<HEX, P, V, real, ELEM_THREADS>), grid, block, 0, s, a, ph); break; \
@@ -55,6 +65,14 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
       default: return -3;                                                                       \
     }                                                                                           \
     return hipGetLastError() == hipSuccess ? 0 : -1;                                            \
+  }
+
+#define FSG_MFMA_CASE(T, P, V, real)                                                           \
+  case T: {                                                                                     \
+    constexpr int EB = MfmaShape<T, P, V, real>::EB;                                            \
+    dim3 grid((unsigned)((count + EB - 1) / EB)), block(256);                                   \
+    hipLaunchKernelGGL((element_mfma_kernel<T, P, V, real>), grid, block, 0, s, a, ph, count);  \
+    break;                                                                                      \
   }
 
 #define FSG_FACE_CASE(FT, SUB, TO, TN, P, V, real)                                             \

@@ -39,6 +39,7 @@ struct ElemArgs {
   const real* D;          // (Q, 3, N) reference gradients, node fastest
   const real* wq;         // (Q)
   const real* minv;       // (N, N) reference inverse mass
+  const real* minvT;      // (N, N) its transpose
   const real* jinv;       // (E, 9)  [r*3+d] = d xi_r / d x_d
   const real* scale;      // (E) min(1, V/1e-4) / |det J|
   const real* detw;       // (E) |det J|

@@ -49,11 +49,12 @@ struct TypeT {
   int N, Q, NS, NI, NB;
   real wsum;                   // sum of the reference volume weights
   real volsum;                 // sum of the reference subcell volumes
-  const real* p1proj;
-  const real* samples;
+  const real* p1proj;   // (N, N) transposed: P1[i][j] = p1proj[j * N + i]
+  const real* samples;  // (N, NS): basis j at sample point k is samples[j * NS + k]
   const real* wq;
-  const real* P;
-  const real* R;
+  const real* P;        // (64, N)
+  const real* PT;       // (N, 64)
+  const real* RT;       // (64, N): R[i][t] = RT[t * N + i]
   const real* subvol;
   const int32_t* int_o;
   const int32_t* int_n;
@@ -351,10 +352,10 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     real hi0 = 0, hi1 = 0, sc0 = 0, sc1 = 0;
     if (lane < N) {
       real l0 = 0, l1 = 0;
-      const real* row = T.p1proj + (int64_t)lane * N;
       for (int j = 0; j < N; ++j) {
-        l0 += row[j] * sm[w][j][0];
-        l1 += row[j] * sm[w][j][1];
+        const real c = T.p1proj[j * N + lane];   // transposed table: P1[lane][j], coalesced
+        l0 += c * sm[w][j][0];
+        l1 += c * sm[w][j][1];
       }
       const real m0 = sm[w][lane][0], m1 = sm[w][lane][1];
       hi0 = (m0 - l0) * (m0 - l0);
@@ -390,9 +391,8 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
       real u[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) u[v] = real(0);
-      const real* row = T.samples + (int64_t)k * N;
       for (int j = 0; j < N; ++j) {
-        const real c = row[j];
+        const real c = T.samples[(int64_t)j * T.NS + k];
 #pragma unroll
         for (int v = 0; v < V; ++v) u[v] += c * sU[w][j][v];
       }
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(64) void subcell_kernel(Args<real> a, Phys<real> ph
 #pragma unroll
     for (int v = 0; v < V; ++v) sub[v] = real(0);
     for (int m = 0; m < N; ++m) {
-      const real c = T.P[s * N + m];
+      const real c = T.PT[m * NSUB + s];
 #pragma unroll
       for (int v = 0; v < V; ++v) sub[v] += c * sU[m][v];
     }
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(64) void subcell_kernel(Args<real> a, Phys<real> ph
     if (a.visc[b]) {
       const real* Vv = a.visc[b] + e * N * V;
       real pv[V];
-      row_dot<V, real>(T.P + s * N, Vv, N, pv);
+      row_dot<V, real>(T.P + s * N, Vv, N, pv);   // (row read once per lane)
 #pragma unroll
       for (int v = 0; v < V; ++v) fv[v] = fv[v] + h * pv[v];
     }
@@ -641,7 +641,7 @@ __global__ __launch_bounds__(64) void subcell_kernel(Args<real> a, Phys<real> ph
 #pragma unroll
       for (int v = 0; v < V; ++v) raw[v] = real(0);
       for (int t = 0; t < NSUB; ++t) {
-        const real c = T.R[s * NSUB + t];
+        const real c = T.RT[t * N + s];
 #pragma unroll
         for (int v = 0; v < V; ++v) raw[v] += c * sS[t][v];
       }
@@ -726,13 +726,13 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
 #pragma unroll
   for (int v = 0; v < V; ++v) msum[v] = real(0);
   for (int k = lane; k < T.Q; k += 64) {
-    const real* row = T.samples + (int64_t)k * N;
     real u[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) u[v] = real(0);
     for (int j = 0; j < N; ++j) {
+      const real c = T.samples[(int64_t)j * T.NS + k];
 #pragma unroll
-      for (int v = 0; v < V; ++v) u[v] += row[j] * sU[j][v];
+      for (int v = 0; v < V; ++v) u[v] += c * sU[j][v];
     }
 #pragma unroll
     for (int v = 0; v < V; ++v) msum[v] += T.wq[k] * u[v];
@@ -751,12 +751,12 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
 #pragma unroll
       for (int v = 0; v < 5; ++v) u[v] = sU[k][v];
     } else {
-      const real* row = T.samples + (int64_t)(k - N) * N;
 #pragma unroll
       for (int v = 0; v < 5; ++v) u[v] = real(0);
       for (int j = 0; j < N; ++j) {
+        const real c = T.samples[(int64_t)j * T.NS + (k - N)];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) u[v] += row[j] * sU[j][v];
+        for (int v = 0; v < 5; ++v) u[v] += c * sU[j][v];
       }
     }
     rmin = nmin(rmin, u[0]);
@@ -897,7 +897,8 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
     d.samples = static_cast<const real*>(s.samples);
     d.wq = static_cast<const real*>(s.wq);
     d.P = static_cast<const real*>(s.P);
-    d.R = static_cast<const real*>(s.R);
+    d.PT = static_cast<const real*>(s.PT);
+    d.RT = static_cast<const real*>(s.RT);
     d.subvol = static_cast<const real*>(s.subvol);
     d.int_o = s.int_o;
     d.int_n = s.int_n;

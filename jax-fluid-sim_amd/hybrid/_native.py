@@ -29,7 +29,7 @@ class BlockDesc(ctypes.Structure):
     _fields_ = [("element_type", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("count", ctypes.c_int64)] + [
         (k, ctypes.c_void_p) for k in ("Bt", "Dt", "D", "wq", "minv", "jinv", "scale", "detw",
-                                       "rec_ptr", "rec_off", "rec_sup", "rec_nf")]
+                                       "rec_ptr", "rec_off", "rec_sup", "rec_nf", "minvT")]
 
 
 class GroupDesc(ctypes.Structure):
@@ -44,7 +44,7 @@ class GroupDesc(ctypes.Structure):
 
 class MoodType(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in ("N", "Q", "NS", "NI", "NB", "reserved")] + [
-        (k, ctypes.c_void_p) for k in ("p1proj", "samples", "wq", "P", "R", "subvol", "int_o",
+        (k, ctypes.c_void_p) for k in ("p1proj", "samples", "wq", "P", "PT", "RT", "subvol", "int_o",
                                        "int_n", "int_area", "int_dc", "int_ptr", "int_list",
                                        "bnd_face", "bnd_gather", "tgt_ptr", "tgt_list")]
 
@@ -118,6 +118,10 @@ def lib():
         L.fsg_chem_react.restype = ctypes.c_int
         L.fsg_chem_last_error.restype = ctypes.c_char_p
         pp = ctypes.POINTER(vp)
+        L.fsg_mfma_selftest.argtypes = [i32, vp, vp, vp, vp]
+        L.fsg_set_option.argtypes = [vp, i32, i32]
+        L.fsg_set_option.restype = ctypes.c_int
+        L.fsg_mfma_selftest.restype = ctypes.c_int
         L.fsg_mood_create.argtypes = [vp, ctypes.POINTER(MoodParams), ctypes.POINTER(MoodType),
                                       ctypes.POINTER(MoodBlock), ctypes.POINTER(MoodGroup), vp, vp,
                                       ctypes.POINTER(vp)]

@@ -73,7 +73,8 @@ class DeviceMesh:
                 type=b.element_type, count=b.count, N=R["N"], Q=R["Q"],
                 Bt=self._t(R["B"].T), Dt=self._t(R["D"].transpose(1, 2, 0)),
                 D=self._t(R["D"].transpose(0, 2, 1)), wq=self._t(R["weights"]),
-                minv=self._t(R["Minv"]), jinv=self._t(b.jinv.reshape(-1, 9)),
+                minv=self._t(R["Minv"]), minvT=self._t(R["Minv"].T),
+                jinv=self._t(b.jinv.reshape(-1, 9)),
                 scale=self._t(scale), detw=self._t(np.abs(b.det)), volume=vol)
             self.block_tables.append(t)
 
@@ -196,7 +197,7 @@ class DeviceMesh:
             d = blocks[i]
             d.element_type = TYPE_ID[t["type"]]
             d.count = t["count"]
-            for k in ("Bt", "Dt", "D", "wq", "minv", "jinv", "scale", "detw", "rec_ptr",
+            for k in ("Bt", "Dt", "D", "wq", "minv", "minvT", "jinv", "scale", "detw", "rec_ptr",
                       "rec_off", "rec_sup", "rec_nf"):
                 setattr(d, k, t[k].data_ptr())
         groups = (nat.GroupDesc * max(len(self.groups), 1))()
@@ -233,3 +234,8 @@ class DeviceMesh:
                 self.ctx = None
         except Exception:
             pass
+
+    def set_element_path(self, mfma=True):
+        """Element operators on the matrix cores (batched, default) or one
+        workgroup per element on the vector units; same discrete operator."""
+        nat.check(nat.lib().fsg_set_option(self.ctx, 1, int(bool(mfma))))

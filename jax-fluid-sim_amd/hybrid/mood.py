@@ -67,11 +67,12 @@ class MoodTables:
             d = self.types[TYPE_ID[t]]
             d.N, d.Q, d.NS = R["N"], R["Q"], samples.shape[0]
             d.NI, d.NB = len(st.internal_owner), len(st.boundary_owner)
-            d.p1proj = self._r(p1_projector(t, p))
-            d.samples = self._r(samples)
+            d.p1proj = self._r(p1_projector(t, p).T)   # column j holds row j: coalesced
+            d.samples = self._r(samples.T)        # (N, NS): coalesced over points
             d.wq = self._r(R["weights"])
             d.P = self._r(st.P)
-            d.R = self._r(st.R)
+            d.PT = self._r(st.P.T)
+            d.RT = self._r(st.R.T)
             d.subvol = self._r(st.volume)
             d.int_o = self._i(st.internal_owner)
             d.int_n = self._i(st.internal_neighbor)

@@ -188,3 +188,22 @@ def test_in_place_stage_matches_out_of_place():
     dint._stage(V, V, V, 1e-3, 0.0, 1.0, db, True)
     for k in a:
         assert torch.equal(a[k], V[k])
+
+
+@pytest.mark.parametrize("name,p", [("smoke", 2), ("split", 3), ("tgv", 3), ("species", 2)])
+def test_vector_unit_element_path(name, p):
+    """The one-workgroup-per-element path (vector units) computes the same
+    operator as the default batched matrix-core path: both match the oracle."""
+    print(f"case {name} p={p} vector-unit element path", flush=True)
+    ds, dop, db, oop, ob, states = _case(name, p)
+    U = ds.to_device(states)
+    ref = oop.rhs(states, ob)
+    bar = _bar(lambda s: oop.rhs(s, ob), states, FP64_TOL)
+    mfma = _np(dop.rhs(U, db))
+    dop.device_mesh.set_element_path(mfma=False)
+    try:
+        vec = _np(dop.rhs(U, db))
+    finally:
+        dop.device_mesh.set_element_path(mfma=True)
+    assert np.all(rel_l2(vec, ref) <= bar), (rel_l2(vec, ref), bar)
+    assert np.all(rel_l2(mfma, ref) <= bar), (rel_l2(mfma, ref), bar)
