@@ -61,13 +61,14 @@ def measured_traffic(kernel, key):
     return int(m.group(1)) if m else None
 
 
-def run_path(topo, dtype, steps, warmup, dt):
+def run_path(topo, dtype, steps, warmup, dt, element_path="auto"):
     import torch
     from hybrid import HybridCompressibleSolver, HybridNavierStokesOperator, HybridSSPRK3
     from hybrid import _native as nat
     solver = HybridCompressibleSolver(topo, polynomial_degree=3, gamma=GAMMA,
                                       dynamic_viscosity=1.0 / RE, prandtl=PR, dtype=dtype)
     op = HybridNavierStokesOperator(solver)
+    op.device_mesh.set_element_path({"auto": None, "mfma": True, "vector": False}[element_path])
     rk = HybridSSPRK3(op, cfl=0.3)
     U0 = solver.state_blocks_from_primitive_function(tgv_primitive)
     if dt is None:
@@ -309,6 +310,9 @@ def main():
                     help="run one path of the residual benchmark, or only the MOOD or "
                          "chemistry section (for profiling; prints that section's JSON)")
     ap.add_argument("--mood-steps", type=int, default=5)
+    ap.add_argument("--element-path", choices=["auto", "mfma", "vector"], default="auto",
+                    help="element operators on the matrix cores or the vector units "
+                         "(auto: the library default, vector units)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -334,7 +338,7 @@ def main():
     for name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
         if args.only and args.only != name:
             continue
-        r = run_path(topo, dtype, args.steps, args.warmup, dt)
+        r = run_path(topo, dtype, args.steps, args.warmup, dt, args.element_path)
         dt = r["dt"]
         r["rate"] = dof * 3 * args.steps / r["elapsed"]
         res[name] = r

@@ -168,9 +168,10 @@ int fsg_timing_read(fsg_context* ctx, double* group_ms, double* block_ms,
                     int64_t* group_launches, int64_t* block_launches);
 
 /* Context options (no counterpart in the reference).
- * FSG_OPT_ELEMENT_MFMA: 1 (default) runs the element operators as batched
- * matrix-core products, 0 as one workgroup per element on the vector units;
- * both compute the same discrete operator. */
+ * FSG_OPT_ELEMENT_MFMA: 1 runs the element operators as batched matrix-core
+ * products, 0 or -1 (default) as one workgroup per element on the vector
+ * units (the faster of the two on MI355X for the p = 3 hexahedron in both
+ * precisions); both compute the same discrete operator. */
 #define FSG_OPT_ELEMENT_MFMA 1
 int fsg_set_option(fsg_context* ctx, int32_t option, int32_t value);
 

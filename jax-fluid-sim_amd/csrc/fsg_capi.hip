@@ -166,7 +166,8 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.terms = terms;
     a.diag = reinterpret_cast<StageDiag*>(diag);
     long ev = timing_begin(c, int(c->groups.size() + b), s);
-    int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s, c->element_mfma);
+    const int mfma = c->element_mfma > 0 ? 1 : 0;
+    int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s, mfma);
     timing_end(c, ev, s);
     if (rc) return fail(rc, "element launch failed (type %d): %s", d.element_type,
                         hipGetErrorString(hipGetLastError()));
@@ -266,7 +267,7 @@ int fsg_diag_reset(fsg_stage_diag* diag, void* stream) {
 int fsg_set_option(fsg_context* ctx, int32_t option, int32_t value) {
   if (!ctx) return fail(-10, "null argument");
   if (option == FSG_OPT_ELEMENT_MFMA) {
-    ctx->element_mfma = value != 0;
+    ctx->element_mfma = value < 0 ? -1 : (value != 0);
     return 0;
   }
   return fail(-13, "unknown option %d", option);

@@ -114,117 +114,151 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
     }
 
   // -- face-side records: surface terms -> sR, lift right-hand sides -> sB --
+  // Round r adds the r-th record of every element of the batch: within a
+  // round the entries touch distinct (node, column) pairs, so the 256
+  // threads issue their loads independently; rounds are ordered by barriers,
+  // so each node's sum runs in record order (deterministic).
   {
-    const int grp = tid >> 4, gl = tid & 15;   // 16 lanes per element
-    for (int e = grp; e < ne; e += 16) {
-      const int64_t r0 = a.rec_ptr[e0 + e], r1 = a.rec_ptr[e0 + e + 1];
-      const int width = visc ? W : V;
-      for (int64_t r = r0; r < r1; ++r) {
-        const real* rec = a.face_out + a.rec_off[r];
-        const int32_t* sup = a.support + a.rec_sup[r];
-        const int nf = a.rec_nf[r];
-        for (int k = gl; k < nf * width; k += 16) {
-          const int kk = k / width, c = k - kk * width;
-          const int node = sup[kk];
-          const real val = rec[kk * W + c];
-          if (c < V)
-            sR[node * WU + e * V + c] += val;
-          else
-            sB[node * WL + e * 12 + (c - V)] += val;
-        }
+    __shared__ int64_t sRec[EB + 1];
+    __shared__ int sRounds;
+    if (tid <= EB) sRec[tid] = a.rec_ptr[e0 + (tid < ne ? tid : ne)];
+    __syncthreads();
+    if (tid == 0) {
+      int m = 0;
+      for (int e = 0; e < ne; ++e) m = max(m, (int)(sRec[e + 1] - sRec[e]));
+      sRounds = m;
+    }
+    __syncthreads();
+    const int width = visc ? W : V;
+    constexpr int MAXF = (P + 1) * (P + 1);          // support nodes of any face side
+    const int per = MAXF * width;
+    for (int r = 0; r < sRounds; ++r) {
+      for (int idx = tid; idx < ne * per; idx += 256) {
+        const int e = idx / per, k = idx - e * per;
+        const int64_t rr = sRec[e] + r;
+        if (rr >= sRec[e + 1]) continue;
+        const int nf = a.rec_nf[rr];
+        if (k >= nf * width) continue;
+        const int kk = k / width, c = k - kk * width;
+        const int node = a.support[a.rec_sup[rr] + kk];
+        const real val = a.face_out[a.rec_off[rr] + kk * W + c];
+        if (c < V)
+          sR[node * WU + e * V + c] += val;
+        else
+          sB[node * WL + e * 12 + (c - V)] += val;
       }
+      __syncthreads();
     }
   }
-  __syncthreads();
 
   // -- BR2 lift L = scale M^-1 b (hybrid/navier_stokes.py:157-186) ------------
   if (visc) {
-    constexpr int CT4 = (CL + 3) / 4;   // column tiles per wave: wave, wave + 4, ...
-    v4 acc[NR * CT4];
+    // wave w: row tile w / CGL, column tiles of group w % CGL (A shared by the group)
+    constexpr int CGL = NR >= 4 ? 1 : 4 / NR, GL = (CL + CGL - 1) / CGL;
+    const int rt = wave / CGL, cg = wave - rt * CGL;
+    v4 acc[GL];
 #pragma unroll
-    for (int rt = 0; rt < NR; ++rt)
+    for (int g = 0; g < GL; ++g) acc[g] = zero4<real>();
+    if (rt < NR) {
 #pragma unroll
-      for (int ci = 0; ci < CT4; ++ci) {
-        const int ct = wave + 4 * ci;
-        v4 c = zero4<real>();
-        if (ct < CL) {
-          for (int k0 = 0; k0 < NK; k0 += 4) {
-            const int i = rt * 16 + (lane & 15), k = k0 + (lane >> 4);
-            const real av = (i < N && k < N) ? a.minvT[k * N + i] : real(0);
-            const real bv = sB[(k0 + (lane >> 4)) * WL + ct * 16 + (lane & 15)];
-            c = mma(av, bv, c);
-          }
+      for (int k0 = 0; k0 < NK; k0 += 4) {
+        const int i = rt * 16 + (lane & 15), k = k0 + (lane >> 4);
+        const real av = (i < N && k < N) ? a.minvT[k * N + i] : real(0);
+#pragma unroll
+        for (int g = 0; g < GL; ++g) {
+          const int ct = cg * GL + g;
+          if (ct < CL) acc[g] = mma(av, sB[k * WL + ct * 16 + (lane & 15)], acc[g]);
         }
-        acc[rt * CT4 + ci] = c;
       }
+    }
     __syncthreads();
+    if (rt < NR) {
 #pragma unroll
-    for (int rt = 0; rt < NR; ++rt)
-#pragma unroll
-      for (int ci = 0; ci < CT4; ++ci) {
-        const int ct = wave + 4 * ci;
+      for (int g = 0; g < GL; ++g) {
+        const int ct = cg * GL + g;
         if (ct >= CL) continue;
-        const v4 c = acc[rt * CT4 + ci];
         const int col = ct * 16 + (lane & 15), e = col / 12;
         const real s = e < EB ? sS[e] : real(0);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int i = rt * 16 + crow<real>(lane, j);
-          if (i < NK) sB[i * WL + col] = i < N ? s * c[j] : real(0);
+          if (i < NK) sB[i * WL + col] = i < N ? s * acc[g][j] : real(0);
         }
       }
+    }
     __syncthreads();
   }
 
   // -- volume points, in chunks of 16 ----------------------------------------
-  constexpr int DT = (NR * CU + 3) / 4;   // divergence tiles per wave
-  v4 div[DT];
+  // divergence / inverse-mass units: wave w -> row tile w / CGD, column group w % CGD
+  constexpr int CGD = NR >= 4 ? 1 : 4 / NR, GD = (CU + CGD - 1) / CGD;
+  const int drt = wave / CGD, dcg = wave - drt * CGD;
+  v4 div[GD];
 #pragma unroll
-  for (int t = 0; t < DT; ++t) div[t] = zero4<real>();
+  for (int g = 0; g < GD; ++g) div[g] = zero4<real>();
+  // chunk products against the basis rows (interpolation, lift at the points):
+  // wave w takes tiles w, w + 4, ... of [CU interpolation | CL lift]
+  constexpr int BTMAX = CU + CL, MB = (BTMAX + 3) / 4;
 
   for (int qc = 0; qc < QC; ++qc) {
-    // interpolants: tiles [interp CU | grad 3 CV | lift CL], round-robin over waves
-    const int ntile = CU + (visc ? 3 * CV + CL : 0);
-    for (int t = wave; t < ntile; t += 4) {
-      v4 c = zero4<real>();
-      const int q = qc * 16 + (lane & 15);
-      if (t < CU) {
-        const int col = t * 16 + (lane & 15);
-        for (int k0 = 0; k0 < NK; k0 += 4) {
-          const int k = k0 + (lane >> 4);
-          const real av = (q < Q && k < N) ? a.Bt[k * Q + q] : real(0);
-          const int kb = k0 + (lane >> 4);
-          const real bv = sU[kb * WU + col] - sA[col];
-          c = mma(av, bv, c);
-        }
+    const int q = qc * 16 + (lane & 15);
+    {
+      const int BT = CU + (visc ? CL : 0);
+      v4 acc[MB];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int ql = crow<real>(lane, j);
-          sQ[ql * WU + col] = sA[col] + c[j];
-        }
-      } else if (t < CU + 3 * CV) {
-        const int u = t - CU, r = u / CV, ct = u - r * CV;
-        const int col = ct * 16 + (lane & 15);
-        for (int k0 = 0; k0 < NK; k0 += 4) {
-          const int k = k0 + (lane >> 4);
-          const real av = (q < Q && k < N) ? a.Dt[(k * 3 + r) * Q + q] : real(0);
-          const real bv = sV[(k0 + (lane >> 4)) * WV + col];
-          c = mma(av, bv, c);
-        }
+      for (int m = 0; m < MB; ++m) acc[m] = zero4<real>();
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sG[(r * 16 + crow<real>(lane, j)) * WV + col] = c[j];
-      } else {
-        const int ct = t - CU - 3 * CV;
-        const int col = ct * 16 + (lane & 15);
-        for (int k0 = 0; k0 < NK; k0 += 4) {
-          const int k = k0 + (lane >> 4);
-          const real av = (q < Q && k < N) ? a.Bt[k * Q + q] : real(0);
-          const real bv = sB[(k0 + (lane >> 4)) * WL + col];
-          c = mma(av, bv, c);
-        }
+      for (int k0 = 0; k0 < NK; k0 += 4) {
+        const int k = k0 + (lane >> 4);
+        const real av = (q < Q && k < N) ? a.Bt[k * Q + q] : real(0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sGL[(crow<real>(lane, j)) * WL + col] = c[j];
+        for (int m = 0; m < MB; ++m) {
+          const int t = wave + 4 * m;
+          if (t < BT) {
+            real bv;
+            if (t < CU) {
+              const int col = t * 16 + (lane & 15);
+              bv = sU[k * WU + col] - sA[col];
+            } else {
+              bv = sB[k * WL + (t - CU) * 16 + (lane & 15)];
+            }
+            acc[m] = mma(av, bv, acc[m]);
+          }
+        }
       }
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const int t = wave + 4 * m;
+        if (t >= BT) continue;
+        if (t < CU) {
+          const int col = t * 16 + (lane & 15);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) sQ[crow<real>(lane, j) * WU + col] = sA[col] + acc[m][j];
+        } else {
+          const int col = (t - CU) * 16 + (lane & 15);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) sGL[crow<real>(lane, j) * WL + col] = acc[m][j];
+        }
+      }
+    }
+    if (visc && wave < 3) {
+      // raw reference gradient along direction r = wave
+      const int r = wave;
+      v4 acc[CV];
+#pragma unroll
+      for (int c = 0; c < CV; ++c) acc[c] = zero4<real>();
+#pragma unroll
+      for (int k0 = 0; k0 < NK; k0 += 4) {
+        const int k = k0 + (lane >> 4);
+        const real av = (q < Q && k < N) ? a.Dt[(k * 3 + r) * Q + q] : real(0);
+#pragma unroll
+        for (int c = 0; c < CV; ++c) acc[c] = mma(av, sV[k * WV + c * 16 + (lane & 15)], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < CV; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          sG[(r * 16 + crow<real>(lane, j)) * WV + c * 16 + (lane & 15)] = acc[c][j];
     }
     __syncthreads();
 
@@ -294,35 +328,33 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
     __syncthreads();
 
     // R += D^T F^ over this chunk: K = 48 = 16 points x 3 directions
+    if (drt < NR) {
+      const int i = drt * 16 + (lane & 15);
 #pragma unroll
-    for (int t = 0; t < DT; ++t) {
-      const int tile = wave + 4 * t;
-      if (tile < NR * CU) {
-        const int rt = tile / CU, ct = tile - rt * CU;
-        const int i = rt * 16 + (lane & 15), col = ct * 16 + (lane & 15);
-        v4 c = div[t];
-        for (int k0 = 0; k0 < 48; k0 += 4) {
-          const int k = k0 + (lane >> 4), ql = k / 3, r = k - ql * 3, q = qc * 16 + ql;
-          const real av = (q < Q && i < N) ? a.D[(q * 3 + r) * N + i] : real(0);
-          const real bv = sF[k * WU + col];
-          c = mma(av, bv, c);
+      for (int k0 = 0; k0 < 48; k0 += 4) {
+        const int k = k0 + (lane >> 4), ql = k / 3, r = k - ql * 3, qq = qc * 16 + ql;
+        const real av = (qq < Q && i < N) ? a.D[(qq * 3 + r) * N + i] : real(0);
+#pragma unroll
+        for (int g = 0; g < GD; ++g) {
+          const int ct = dcg * GD + g;
+          if (ct < CU) div[g] = mma(av, sF[k * WU + ct * 16 + (lane & 15)], div[g]);
         }
-        div[t] = c;
       }
     }
     __syncthreads();
   }
 
   // -- residual: R = volume terms + surface terms ----------------------------
+  if (drt < NR) {
 #pragma unroll
-  for (int t = 0; t < DT; ++t) {
-    const int tile = wave + 4 * t;
-    if (tile < NR * CU) {
-      const int rt = tile / CU, ct = tile - rt * CU, col = ct * 16 + (lane & 15);
+    for (int g = 0; g < GD; ++g) {
+      const int ct = dcg * GD + g;
+      if (ct >= CU) continue;
+      const int col = ct * 16 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int i = rt * 16 + crow<real>(lane, j);
-        if (i < NK) sR[i * WU + col] += div[t][j];
+        const int i = drt * 16 + crow<real>(lane, j);
+        if (i < NK) sR[i * WU + col] += div[g][j];
       }
     }
   }
@@ -337,41 +369,42 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
   }
 
   // -- rhs = scale M^-1 R, then the stage epilogue -----------------------------
+  if (drt < NR) {
 #pragma unroll
-  for (int t = 0; t < DT; ++t) {
-    const int tile = wave + 4 * t;
-    if (tile >= NR * CU) continue;
-    const int rt = tile / CU, ct = tile - rt * CU;
-    const int col = ct * 16 + (lane & 15);
-    v4 c = zero4<real>();
+    for (int g = 0; g < GD; ++g) div[g] = zero4<real>();
+    const int i = drt * 16 + (lane & 15);
+#pragma unroll
     for (int k0 = 0; k0 < NK; k0 += 4) {
-      const int i = rt * 16 + (lane & 15), k = k0 + (lane >> 4);
+      const int k = k0 + (lane >> 4);
       const real av = (i < N && k < N) ? a.minvT[k * N + i] : real(0);
-      const real bv = sR[(k0 + (lane >> 4)) * WU + col];
-      c = mma(av, bv, c);
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        const int ct = dcg * GD + g;
+        if (ct < CU) div[g] = mma(av, sR[k * WU + ct * 16 + (lane & 15)], div[g]);
+      }
     }
-    div[t] = c;
   }
   __syncthreads();   // every wave has read sR: it now holds the new state
+  if (drt < NR) {
 #pragma unroll
-  for (int t = 0; t < DT; ++t) {
-    const int tile = wave + 4 * t;
-    if (tile >= NR * CU) continue;
-    const int rt = tile / CU, ct = tile - rt * CU;
-    const int col = ct * 16 + (lane & 15), e = col / V, v = col - e * V;
+    for (int g = 0; g < GD; ++g) {
+      const int ct = dcg * GD + g;
+      if (ct >= CU) continue;
+      const int col = ct * 16 + (lane & 15), e = col / V, v = col - e * V;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = rt * 16 + crow<real>(lane, j);
-      if (i >= N || e >= ne) continue;
-      const real rhs = sS[e] * div[t][j];
-      const int64_t g = ((e0 + e) * N + i) * V + v;
-      if (a.mode == 1) {
-        a.out[g] = rhs;
-      } else {
-        const real dte = a.dt_elem ? a.dt_elem[e0 + e] : a.dt;
-        const real nxt = a.alpha * a.Ubase[g] + a.beta * (sU[i * WU + col] + dte * rhs);
-        a.out[g] = nxt;
-        sR[i * WU + col] = nxt;
+      for (int j = 0; j < 4; ++j) {
+        const int i = drt * 16 + crow<real>(lane, j);
+        if (i >= N || e >= ne) continue;
+        const real rhs = sS[e] * div[g][j];
+        const int64_t gi = ((e0 + e) * N + i) * V + v;
+        if (a.mode == 1) {
+          a.out[gi] = rhs;
+        } else {
+          const real dte = a.dt_elem ? a.dt_elem[e0 + e] : a.dt;
+          const real nxt = a.alpha * a.Ubase[gi] + a.beta * (sU[i * WU + col] + dte * rhs);
+          a.out[gi] = nxt;
+          sR[i * WU + col] = nxt;
+        }
       }
     }
   }

@@ -110,6 +110,7 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   __shared__ real sb[N * 12];
   __shared__ real sL[N * 12];
   __shared__ real sF[Q * 3 * V];
+  __shared__ real sP[2 * N * V];
   __shared__ double red[2][NT / 64];
   __shared__ unsigned int redn[NT / 64];
 
@@ -155,13 +156,20 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
 #pragma unroll
   for (int k = 0; k < 9; ++k) J[k] = a.jinv[e * 9 + k];
 
-  // BR2 global lift r = M_phys^-1 b  (hybrid/navier_stokes.py:157-186).
+  // BR2 global lift r = M_phys^-1 b  (hybrid/navier_stokes.py:157-186): two
+  // threads per node, six columns each, so each row of M^-1 is read once per
+  // thread rather than once per output.
   if (visc) {
-    for (int k = tid; k < N * 12; k += NT) {
-      int i = k / 12, c = k - i * 12;
-      real acc = real(0);
-      for (int j = 0; j < N; ++j) acc += a.minv[i * N + j] * sb[j * 12 + c];
-      sL[k] = s * acc;
+    for (int k = tid; k < N * 2; k += NT) {
+      const int i = k >> 1, c0 = (k & 1) * 6;
+      real acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int j = 0; j < N; ++j) {
+        const real m = a.minv[i * N + j];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += m * sb[j * 12 + c0 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sL[i * 12 + c0 + c] = s * acc[c];
     }
   }
   __syncthreads();
@@ -246,15 +254,27 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   }
   __syncthreads();
 
-  // R_i += sum_q sum_r dphi_i/dxi_r F^_r
-  for (int k = tid; k < N * V; k += NT) {
-    int i = k / V, v = k - i * V;
-    real acc = real(0);
-    for (int q = 0; q < Q; ++q)
+  // R_i += sum_q sum_r dphi_i/dxi_r F^_r: two threads per node, each over
+  // half of the points and all variables (one read of D per point and
+  // direction), partial sums combined through LDS.
+  for (int k = tid; k < N * 2; k += NT) {
+    const int i = k >> 1, h = k & 1;
+    const int q0 = h ? Q / 2 : 0, q1 = h ? Q : Q / 2;
+    real acc[V];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) acc += a.D[(q * 3 + r) * N + i] * sF[(q * 3 + r) * V + v];
-    sR[k] += acc;
+    for (int v = 0; v < V; ++v) acc[v] = real(0);
+    for (int q = q0; q < q1; ++q)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const real d = a.D[(q * 3 + r) * N + i];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += d * sF[(q * 3 + r) * V + v];
+      }
+#pragma unroll
+    for (int v = 0; v < V; ++v) sP[(h * N + i) * V + v] = acc[v];
   }
+  __syncthreads();
+  for (int k = tid; k < N * V; k += NT) sR[k] += sP[k] + sP[N * V + k];
   __syncthreads();
 
   real* out = a.out + e * (N * V);
@@ -263,17 +283,27 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
     return;
   }
   const real dte = a.dt_elem ? a.dt_elem[e] : a.dt;
-  for (int k = tid; k < N * V; k += NT) {
-    int i = k / V, v = k - i * V;
-    real acc = real(0);
-    for (int j = 0; j < N; ++j) acc += a.minv[i * N + j] * sR[j * V + v];
-    real rhs = s * acc;
-    if (a.mode == 1) {
-      out[k] = rhs;
-    } else {
-      real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (sU[k] + dte * rhs);
-      out[k] = nxt;
-      sF[k] = nxt;
+  // rhs = scale M^-1 R: one thread per node, all variables per row read.
+  for (int i = tid; i < N; i += NT) {
+    real acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = real(0);
+    for (int j = 0; j < N; ++j) {
+      const real m = a.minv[i * N + j];
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] += m * sR[j * V + v];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = i * V + v;
+      const real rhs = s * acc[v];
+      if (a.mode == 1) {
+        out[k] = rhs;
+      } else {
+        const real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (sU[k] + dte * rhs);
+        out[k] = nxt;
+        sF[k] = nxt;
+      }
     }
   }
   if (a.mode != 2) return;
@@ -466,78 +496,162 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   __syncthreads();
 
   if (visc) {
-    // Phase 2: lift right-hand sides b = 1/2 sum_q w [[v]] (x) n phi.
-    for (int k = tid; k < FO * 4; k += 64) {
-      int kk = k >> 2, m = k & 3;
-      real acc = real(0);
-      for (int q = 0; q < QF; ++q) acc += tbo[q * FO + kk] * sJ[q * 4 + m];
+    // Phase 2: lift right-hand sides b = 1/2 sum_q w [[v]] (x) n phi.  Four
+    // lanes per support node split the points; each table entry is read once.
+    {
+      const int kk = tid >> 2, part = tid & 3;
+      real acc_o[4] = {0, 0, 0, 0}, acc_n[4] = {0, 0, 0, 0};
+      for (int q = part; q < QF; q += 4) {
+        if (kk < FO) {
+          const real c = tbo[q * FO + kk];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) sbo[kk * 12 + m * 3 + d] = real(0.5) * acc * n[d];
-    }
-    if constexpr (NBR) {
-      for (int k = tid; k < FN * 4; k += 64) {
-        int kk = k >> 2, m = k & 3;
-        real acc = real(0);
-        for (int q = 0; q < QF; ++q) acc += tbn[q * FN + kk] * sJ[q * 4 + m];
+          for (int m = 0; m < 4; ++m) acc_o[m] += c * sJ[q * 4 + m];
+        }
+        if constexpr (NBR) {
+          if (kk < FN) {
+            const real c = tbn[q * FN + kk];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) sbn[kk * 12 + m * 3 + d] = real(0.5) * acc * n[d];
+            for (int m = 0; m < 4; ++m) acc_n[m] += c * sJ[q * 4 + m];
+          }
+        }
       }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        acc_o[m] += __shfl_xor(acc_o[m], 1, 64);
+        acc_o[m] += __shfl_xor(acc_o[m], 2, 64);
+        acc_n[m] += __shfl_xor(acc_n[m], 1, 64);
+        acc_n[m] += __shfl_xor(acc_n[m], 2, 64);
+      }
+      static_assert(FO <= 16 && FN <= 16, "four lanes per support node in one wave");
+      if (part == 0 && kk < FO)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) sbo[kk * 12 + m * 3 + d] = real(0.5) * acc_o[m] * n[d];
+      if constexpr (NBR)
+        if (part == 0 && kk < FN)
+#pragma unroll
+          for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sbn[kk * 12 + m * 3 + d] = real(0.5) * acc_n[m] * n[d];
     }
     __syncthreads();
 
     // Phase 3: face gradients = raw polynomial gradient + eta * local lift.
-    const real so = a.scale_o[eo];
-    real Jo[9];
+    // FP32: four lanes per face point split the element nodes and the support
+    // nodes (the gradient table is read once per (point, node)); FP64 keeps one
+    // lane per (point, variable), which needs fewer registers per lane.
+    if constexpr (sizeof(real) == 4) {
+      static_assert(QF <= 16 || QF == 25, "face points per wave");
+      auto side = [&](const real* tb, const real* sv, const real* sbb, const real* Jx, real sc,
+                      int NE, int NF, real* sg) {
+        for (int q0 = 0; q0 < QF; q0 += 16) {
+          const int q = q0 + (tid >> 2), part = tid & 3;
+          real G[12], L[12];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
-    for (int k = tid; k < QF * 4; k += 64) {
-      int q = k >> 2, m = k & 3;
-      const real* dph = tbo + QF * FO * 2 + q * NO * 3;
-      real G0 = 0, G1 = 0, G2 = 0;
-      for (int i = 0; i < NO; ++i) {
-        real vm = svo[i * 4 + m];
-        G0 += dph[i * 3 + 0] * vm;
-        G1 += dph[i * 3 + 1] * vm;
-        G2 += dph[i * 3 + 2] * vm;
+          for (int k = 0; k < 12; ++k) {
+            G[k] = real(0);
+            L[k] = real(0);
+          }
+          if (q < QF) {
+            const real* dph = tb + QF * NF * 2 + q * NE * 3;
+            for (int i = part; i < NE; i += 4) {
+              const real d0 = dph[i * 3 + 0], d1 = dph[i * 3 + 1], d2 = dph[i * 3 + 2];
+#pragma unroll
+              for (int m = 0; m < 4; ++m) {
+                const real vm = sv[i * 4 + m];
+                G[m * 3 + 0] += d0 * vm;
+                G[m * 3 + 1] += d1 * vm;
+                G[m * 3 + 2] += d2 * vm;
+              }
+            }
+            const real* lop = tb + QF * NF + q * NF;
+            for (int kk = part; kk < NF; kk += 4) {
+              const real c = lop[kk];
+#pragma unroll
+              for (int k = 0; k < 12; ++k) L[k] += c * sbb[kk * 12 + k];
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 12; ++k) {
+            G[k] += __shfl_xor(G[k], 1, 64);
+            G[k] += __shfl_xor(G[k], 2, 64);
+            L[k] += __shfl_xor(L[k], 1, 64);
+            L[k] += __shfl_xor(L[k], 2, 64);
+          }
+          if (q < QF && part == 0)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                sg[q * 12 + m * 3 + d] = G[m * 3 + 0] * Jx[0 * 3 + d] + G[m * 3 + 1] * Jx[1 * 3 + d] +
+                                         G[m * 3 + 2] * Jx[2 * 3 + d] + ph.eta * (sc * L[m * 3 + d]);
+        }
+      };
+      real Jo[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
+      side(tbo, svo, sbo, Jo, a.scale_o[eo], NO, FO, sgo);
+      if constexpr (NBR) {
+        real Jn[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
+        side(tbn, svn, sbn, Jn, a.scale_n[en], NN, FN, sgn);
       }
-      const real* lop = tbo + QF * FO + q * FO;
-      real L[3] = {0, 0, 0};
-      for (int kk = 0; kk < FO; ++kk) {
-        real c = lop[kk];
+    } else {
+      const real so = a.scale_o[eo];
+      real Jo[9];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) L[d] += c * sbo[kk * 12 + m * 3 + d];
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        sgo[q * 12 + m * 3 + d] = G0 * Jo[0 * 3 + d] + G1 * Jo[1 * 3 + d] + G2 * Jo[2 * 3 + d] +
-                                  ph.eta * (so * L[d]);
-    }
-    if constexpr (NBR) {
-      const real sn = a.scale_n[en];
-      real Jn[9];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
+      for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
       for (int k = tid; k < QF * 4; k += 64) {
         int q = k >> 2, m = k & 3;
-        const real* dph = tbn + QF * FN * 2 + q * NN * 3;
+        const real* dph = tbo + QF * FO * 2 + q * NO * 3;
         real G0 = 0, G1 = 0, G2 = 0;
-        for (int i = 0; i < NN; ++i) {
-          real vm = svn[i * 4 + m];
+        for (int i = 0; i < NO; ++i) {
+          real vm = svo[i * 4 + m];
           G0 += dph[i * 3 + 0] * vm;
           G1 += dph[i * 3 + 1] * vm;
           G2 += dph[i * 3 + 2] * vm;
         }
-        const real* lop = tbn + QF * FN + q * FN;
+        const real* lop = tbo + QF * FO + q * FO;
         real L[3] = {0, 0, 0};
-        for (int kk = 0; kk < FN; ++kk) {
+        for (int kk = 0; kk < FO; ++kk) {
           real c = lop[kk];
 #pragma unroll
-          for (int d = 0; d < 3; ++d) L[d] += c * sbn[kk * 12 + m * 3 + d];
+          for (int d = 0; d < 3; ++d) L[d] += c * sbo[kk * 12 + m * 3 + d];
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-          sgn[q * 12 + m * 3 + d] = G0 * Jn[0 * 3 + d] + G1 * Jn[1 * 3 + d] +
-                                    G2 * Jn[2 * 3 + d] + ph.eta * (sn * L[d]);
+          sgo[q * 12 + m * 3 + d] = G0 * Jo[0 * 3 + d] + G1 * Jo[1 * 3 + d] + G2 * Jo[2 * 3 + d] +
+                                    ph.eta * (so * L[d]);
+      }
+      if constexpr (NBR) {
+        const real sn = a.scale_n[en];
+        real Jn[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
+        for (int k = tid; k < QF * 4; k += 64) {
+          int q = k >> 2, m = k & 3;
+          const real* dph = tbn + QF * FN * 2 + q * NN * 3;
+          real G0 = 0, G1 = 0, G2 = 0;
+          for (int i = 0; i < NN; ++i) {
+            real vm = svn[i * 4 + m];
+            G0 += dph[i * 3 + 0] * vm;
+            G1 += dph[i * 3 + 1] * vm;
+            G2 += dph[i * 3 + 2] * vm;
+          }
+          const real* lop = tbn + QF * FN + q * FN;
+          real L[3] = {0, 0, 0};
+          for (int kk = 0; kk < FN; ++kk) {
+            real c = lop[kk];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) L[d] += c * sbn[kk * 12 + m * 3 + d];
+          }
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            sgn[q * 12 + m * 3 + d] = G0 * Jn[0 * 3 + d] + G1 * Jn[1 * 3 + d] +
+                                      G2 * Jn[2 * 3 + d] + ph.eta * (sn * L[d]);
+        }
       }
     }
     __syncthreads();
@@ -565,26 +679,51 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     __syncthreads();
   }
 
-  // Phase 5: project onto the face-support basis of each side.
-  real* oo = a.out_o + f * (FO * W);
-  for (int k = tid; k < FO * V; k += 64) {
-    int kk = k / V, v = k - kk * V;
-    real acc = real(0);
-    for (int q = 0; q < QF; ++q) acc += tbo[q * FO + kk] * sSo[q * V + v];
-    oo[kk * W + v] = acc;
-  }
-  if (visc)
-    for (int k = tid; k < FO * 12; k += 64) oo[(k / 12) * W + V + (k % 12)] = sbo[k];
-  if constexpr (NBR) {
-    real* on = a.out_n + f * (FN * W);
-    for (int k = tid; k < FN * V; k += 64) {
-      int kk = k / V, v = k - kk * V;
-      real acc = real(0);
-      for (int q = 0; q < QF; ++q) acc += tbn[q * FN + kk] * sSn[q * V + v];
-      on[kk * W + v] = acc;
+  // Phase 5: project onto the face-support basis of each side: four lanes per
+  // support node split the points.
+  {
+    const int kk = tid >> 2, part = tid & 3;
+    real ao[V], an[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      ao[v] = real(0);
+      an[v] = real(0);
     }
+    for (int q = part; q < QF; q += 4) {
+      if (kk < FO) {
+        const real c = tbo[q * FO + kk];
+#pragma unroll
+        for (int v = 0; v < V; ++v) ao[v] += c * sSo[q * V + v];
+      }
+      if constexpr (NBR) {
+        if (kk < FN) {
+          const real c = tbn[q * FN + kk];
+#pragma unroll
+          for (int v = 0; v < V; ++v) an[v] += c * sSn[q * V + v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      ao[v] += __shfl_xor(ao[v], 1, 64);
+      ao[v] += __shfl_xor(ao[v], 2, 64);
+      an[v] += __shfl_xor(an[v], 1, 64);
+      an[v] += __shfl_xor(an[v], 2, 64);
+    }
+    real* oo = a.out_o + f * (FO * W);
+    if (part == 0 && kk < FO)
+#pragma unroll
+      for (int v = 0; v < V; ++v) oo[kk * W + v] = ao[v];
     if (visc)
-      for (int k = tid; k < FN * 12; k += 64) on[(k / 12) * W + V + (k % 12)] = sbn[k];
+      for (int k = tid; k < FO * 12; k += 64) oo[(k / 12) * W + V + (k % 12)] = sbo[k];
+    if constexpr (NBR) {
+      real* on = a.out_n + f * (FN * W);
+      if (part == 0 && kk < FN)
+#pragma unroll
+        for (int v = 0; v < V; ++v) on[kk * W + v] = an[v];
+      if (visc)
+        for (int k = tid; k < FN * 12; k += 64) on[(k / 12) * W + V + (k % 12)] = sbn[k];
+    }
   }
 }
 

@@ -235,7 +235,8 @@ class DeviceMesh:
         except Exception:
             pass
 
-    def set_element_path(self, mfma=True):
-        """Element operators on the matrix cores (batched, default) or one
-        workgroup per element on the vector units; same discrete operator."""
-        nat.check(nat.lib().fsg_set_option(self.ctx, 1, int(bool(mfma))))
+    def set_element_path(self, mfma=None):
+        """Element operators as batched matrix-core products (True) or one
+        workgroup per element on the vector units (False, or None: the
+        default); the discrete operator is the same."""
+        nat.check(nat.lib().fsg_set_option(self.ctx, 1, -1 if mfma is None else int(bool(mfma))))

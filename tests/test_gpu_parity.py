@@ -191,19 +191,25 @@ def test_in_place_stage_matches_out_of_place():
 
 
 @pytest.mark.parametrize("name,p", [("smoke", 2), ("split", 3), ("tgv", 3), ("species", 2)])
-def test_vector_unit_element_path(name, p):
-    """The one-workgroup-per-element path (vector units) computes the same
-    operator as the default batched matrix-core path: both match the oracle."""
-    print(f"case {name} p={p} vector-unit element path", flush=True)
-    ds, dop, db, oop, ob, states = _case(name, p)
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_both_element_paths(name, p, dtype):
+    """The batched matrix-core element path and the one-workgroup-per-element
+    vector-unit path compute the same operator: both match the oracle, in
+    FP64 at the FP64 bar and in FP32 at the FP32 bar."""
+    print(f"case {name} p={p} {dtype} both element paths", flush=True)
+    ds, dop, db, oop, ob, states = _case(name, p, dtype=dtype)
     U = ds.to_device(states)
     ref = oop.rhs(states, ob)
-    bar = _bar(lambda s: oop.rhs(s, ob), states, FP64_TOL)
-    mfma = _np(dop.rhs(U, db))
-    dop.device_mesh.set_element_path(mfma=False)
+    if dtype == torch.float64:
+        bar = _bar(lambda s: oop.rhs(s, ob), states, FP64_TOL)
+    else:
+        bar = FP32_TOL.get(name, 5e-5) if name != "smoke" else 5e-5
+    out = {}
     try:
-        vec = _np(dop.rhs(U, db))
+        for mfma in (True, False):
+            dop.device_mesh.set_element_path(mfma=mfma)
+            out[mfma] = _np(dop.rhs(U, db))
     finally:
-        dop.device_mesh.set_element_path(mfma=True)
-    assert np.all(rel_l2(vec, ref) <= bar), (rel_l2(vec, ref), bar)
-    assert np.all(rel_l2(mfma, ref) <= bar), (rel_l2(mfma, ref), bar)
+        dop.device_mesh.set_element_path(None)
+    for mfma, got in out.items():
+        assert np.all(rel_l2(got, ref) <= bar), (mfma, rel_l2(got, ref), bar)
