@@ -116,6 +116,34 @@ def shock_bubble_primitive(x):
         np.array([r2, u2, 0.0, 0.0, p2])
 
 
+AIR_Y = np.zeros(11)
+AIR_Y[9], AIR_Y[10] = 0.767, 0.233        # N2, O2 (species order e-, N+, ..., N2, O2)
+
+
+def reacting_inflow_primitive(x, mach=3.0, seed=5, amplitude=0.01):
+    """Uniform Mach 3 stream of air at 1500 K in the coupler's dimensionless
+    units (rho = 1, p = 5, so T = (p / rho) T_ref with T_ref = 300 K) along
+    +x, with a seeded smooth density perturbation of 1 % amplitude (eight
+    random Fourier modes, wavenumbers 1-3, normalised to unit peak) at
+    constant pressure; N2/O2 mass fractions 0.767/0.233.  Returns the
+    (..., 16) primitive field and the inflow farfield."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, 4, size=(8, 3))
+    phase = rng.uniform(0, 2 * np.pi, 8)
+    amp = rng.uniform(0.5, 1.0, 8)
+    field = sum(a * np.sin(2 * np.pi * (x @ kk) + ph) for a, kk, ph in zip(amp, k, phase))
+    rho = 1.0 + amplitude * field / amp.sum()
+    p = 5.0
+    u = mach * np.sqrt(GAMMA * p / 1.0)
+    W = np.zeros(x.shape[:-1] + (16,))
+    W[..., 0] = rho
+    W[..., 1] = u
+    W[..., 4] = p
+    W[..., 5:] = AIR_Y
+    far = np.concatenate([[1.0, u, 0.0, 0.0, p], AIR_Y])
+    return W, far
+
+
 def mood_rate(n=64, split=0.2, steps=5, seed=1):
     """Shock-bubble on the unit box of n^3 hexahedra with a seeded fraction
     split into wedges / tetrahedra, p=2, HLLC + BR2 (Re 1e4 on the post-shock
@@ -177,6 +205,148 @@ def mood_rate(n=64, split=0.2, steps=5, seed=1):
     return out
 
 
+def reacting_rate(n=86, split=0.2, steps=5, seed=1, train_cells=1 << 15, train_traj=24,
+                  train_steps=3000):
+    """Reacting hybrid-mesh flow: the shock-bubble mesh family scaled up (unit
+    box of n^3 hexahedra, a seeded 20 % split into wedges / tetrahedra; n = 86
+    gives about 1.0M elements), p=2, 5 + 11 variables, Mach 3 air at 1500 K
+    with a 1 % density perturbation (Re 1e4 on the inflow speed), MOOD +
+    positivity active, and the Park-11 coupler after every accepted SSP-RK3
+    step (operator splitting as in run_hybrid_sphere.py:280-312).
+    Plain path: FP64 residual, FP64 point-implicit chemistry (12 Newton
+    iterations, temperature re-solved inside).  Fast path: FP32 residual and
+    the BF16-MFMA surrogate of the same substep with its integrator fallback,
+    the surrogate trained here on integrator trajectories started from the
+    flow's own cells.  Same dt for both; a timed step is the RK step and the
+    chemistry."""
+    import torch
+    from hybrid import (FarfieldWallBoundary, HybridCompressibleSolver, HybridMOODSubcellLimiter,
+                        HybridNavierStokesOperator, HybridPositivityLimiter, HybridSSPRK3,
+                        box_mesh)
+    from hybrid import _native as nat
+    from hybrid.chemistry import Air11ParkMechanism, ReactingFlowCoupler
+    from hybrid.chemistry.air11_park import SPECIES_NAMES
+    from hybrid.chemistry.surrogate import ChemistrySurrogate
+    from hybrid.topology import Topology
+    import ctypes
+    names = {"xmin": "inlet", "xmax": "outlet", "ymin": "walls", "ymax": "walls",
+             "zmin": "walls", "zmax": "walls"}
+    t0 = time.perf_counter()
+    mesh = Topology(box_mesh(n, split_fraction=split, seed=seed, plane_names=names))
+    _, far = reacting_inflow_primitive(np.zeros((1, 3)))
+    mech = Air11ParkMechanism()
+    out = {"workload": f"reacting air, unit box {n}^3 hexahedra, {int(split * 100)}% split "
+                       "(seeded, wedges/tetrahedra), p=2, 16 variables, Mach 3 at 1500 K, "
+                       "MOOD + positivity, Park-11 chemistry after every step",
+           "mesh_s": time.perf_counter() - t0}
+    res, dt = {}, None
+    for key, dtype, use_sur in (("f64_plain", torch.float64, False),
+                                ("f32_surrogate", torch.float32, True)):
+        t0 = time.perf_counter()
+        solver = HybridCompressibleSolver(mesh, species_names=SPECIES_NAMES, polynomial_degree=2,
+                                          gamma=GAMMA, dynamic_viscosity=far[1] / 1.0e4,
+                                          prandtl=PR, dtype=dtype)
+        op = HybridNavierStokesOperator(solver)
+        pos = HybridPositivityLimiter(op, density_floor=1e-7, pressure_floor=1e-7)
+        lim = HybridMOODSubcellLimiter(op, threshold=0.05, positivity=pos, reconstruction="weno")
+        rk = HybridSSPRK3(op, cfl=0.3, limiter=lim)
+        bnd = FarfieldWallBoundary(solver, far)
+        U0 = solver.state_blocks_from_primitive_function(
+            lambda x: reacting_inflow_primitive(x)[0])
+        if dt is None:
+            dt = float(rk.stable_dt(U0))
+        coupler = ReactingFlowCoupler(mech, max_newton_iters=12, dimensionless=True, gamma=GAMMA)
+        entry = {}
+        if use_sur:
+            # training data: integrator trajectories from a sample of the
+            # flow's cells, temperature and density spread by +-10 % / +-5 %
+            ts = time.perf_counter()
+            Ud = torch.cat([v.double().reshape(-1, 16) for v in U0.values()])
+            g = torch.Generator(device="cuda").manual_seed(seed)
+            pick = torch.randint(0, Ud.shape[0], (train_cells,), device="cuda", generator=g)
+            rho_s = torch.empty(train_cells, 11, dtype=torch.float64, device="cuda")
+            T0 = torch.empty(train_cells, dtype=torch.float64, device="cuda")
+            p = coupler._params()
+            nat.check_chem(nat.lib().fsg_chem_prepare(
+                nat.ptr(mech.table), nat.ptr(Ud[pick].contiguous()), nat.F64, train_cells,
+                ctypes.byref(p), nat.ptr(rho_s), nat.ptr(T0), nat.stream_handle(Ud.device)))
+            u = lambda a: 1 + a * (2 * torch.rand(train_cells, device="cuda", generator=g,
+                                                  dtype=torch.float64) - 1)
+            T0 = T0 * u(0.10)
+            rho_s = rho_s * u(0.05)[:, None]
+            sur = ChemistrySurrogate(coupler.integrator, constant_temperature=False, seed=seed)
+            X, O, _ = sur.dataset(rho_s, T0, dt * coupler.t_ref, trajectory_steps=train_traj)
+            sur.set_normalisation(X, O)
+            hist = sur.train(X, O, steps=train_steps, batch=4096)
+            torch.cuda.synchronize()
+            coupler.surrogate = sur
+            entry["surrogate_train"] = dict(samples=int(X.shape[0]), steps=train_steps,
+                                            loss_first_last=[hist[0], hist[-1]],
+                                            seconds=time.perf_counter() - ts)
+            del X, O
+        U, valid, _, _ = rk.checked_step(U0, dt, bnd)        # warm-up (builds the MOOD tables)
+        U = coupler.advance(U, dt)
+        torch.cuda.synchronize()
+        entry["setup_s"] = time.perf_counter() - t0
+        chem_ev = []
+        U = U0
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        rejected = fallback = 0
+        for _ in range(steps):
+            cand, valid, rmin, pmin = rk.checked_step(U, dt, bnd)
+            if not bool(valid):
+                rejected += 1
+                continue
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            U = coupler.advance(cand, dt)
+            b.record()
+            chem_ev.append((a, b))
+            if use_sur:
+                fallback += int(coupler.last_fallback.sum())
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        chem_ms = sum(a.elapsed_time(b) for a, b in chem_ev)
+        dof = sum(v.shape[0] * v.shape[1] for v in U.values())
+        E = sum(b.count for b in solver.mesh.blocks)
+        entry.update(ms_per_step=ms / steps, chemistry_ms_per_step=chem_ms / steps,
+                     dof_updates_per_s_per_stage=dof * 3 * steps / (ms * 1e-3),
+                     rejected_steps=rejected, troubled_last_stage=lim.troubled_count())
+        if use_sur:
+            entry["fallback_fraction"] = fallback / max(dof * (steps - rejected), 1)
+        res[key] = (entry, U)
+        out["elements"] = {b.element_type: b.count for b in solver.mesh.blocks}
+        out["elements_total"], out["dof_per_var"] = E, dof
+        print(f"[bench] reacting {key}: {ms / steps:.2f} ms/step (chemistry "
+              f"{chem_ms / steps:.2f}), {E} elements, troubled {entry['troubled_last_stage']}",
+              file=sys.stderr, flush=True)
+        del rk, lim, pos, op, solver, coupler
+        torch.cuda.empty_cache()
+    for key in res:
+        out[key] = res[key][0]
+    a, b = res["f32_surrogate"][1], res["f64_plain"][1]
+    A = torch.cat([a[k].double().reshape(-1, 16) for k in b])
+    B = torch.cat([b[k].reshape(-1, 16) for k in b])
+    diff = torch.linalg.vector_norm(A - B, dim=0)
+    norm = torch.linalg.vector_norm(B, dim=0)
+    labels = ["rho", "rho_u", "rho_v", "rho_w", "rho_E"] + [f"rho_{s}" for s in SPECIES_NAMES]
+    # relative L2 per variable for the variables with O(1) norms; v, w and the
+    # species are measured against the density norm (their own norms are
+    # round-off or tiny in this stream)
+    out["fast_rel_l2_vs_plain_fp64"] = {labels[i]: float(diff[i] / norm[i]) for i in (0, 1, 4)}
+    out["fast_l2_over_rho_l2"] = {k: float(v) for k, v in zip(labels, (diff / norm[0]).cpu())}
+    out["species_rel_l2"] = {f"rho_{s}": float(diff[5 + i] / norm[5 + i])
+                             for i, s in enumerate(SPECIES_NAMES) if s in ("N2", "O2", "O", "NO")}
+    out["speedup_fast_vs_plain"] = (out["f64_plain"]["ms_per_step"]
+                                    / out["f32_surrogate"]["ms_per_step"])
+    out["steps"], out["dt"] = steps, dt
+    return out
+
+
 def _steps(rk, U, dt, bnd, steps):
     for _ in range(steps):
         U, valid, rmin, pmin = rk.checked_step(U, dt, bnd)
@@ -232,6 +402,7 @@ def chemistry_rate(cells, seed=11, train_cells=1 << 18, train_steps=3000):
     print(f"[bench] surrogate trained in {train_s:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}",
           file=sys.stderr, flush=True)
     sur.infer(r[:4096], t[:4096], 1e-7)
+    sur.step(r[:4096], t[:4096], 1e-7)
     (pred, fb), ms_inf = _timed(lambda: sur.infer(r, t, 1e-7))
     (full, _, _), ms_step = _timed(lambda: sur.step(r, t, 1e-7))
     Yref = ref_ct / ref_ct.sum(1, keepdim=True)
@@ -306,10 +477,13 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-chem", action="store_true")
     ap.add_argument("--skip-mood", action="store_true")
-    ap.add_argument("--only", choices=["f64", "f32", "mood", "chem"], default=None,
+    ap.add_argument("--only", choices=["f64", "f32", "mood", "chem", "reacting"], default=None,
                     help="run one path of the residual benchmark, or only the MOOD or "
                          "chemistry section (for profiling; prints that section's JSON)")
     ap.add_argument("--mood-steps", type=int, default=5)
+    ap.add_argument("--skip-reacting", action="store_true")
+    ap.add_argument("--reacting-steps", type=int, default=5)
+    ap.add_argument("--reacting-n", type=int, default=86)
     ap.add_argument("--element-path", choices=["auto", "mfma", "vector"], default="auto",
                     help="element operators on the matrix cores or the vector units "
                          "(auto: the library default, vector units)")
@@ -320,6 +494,9 @@ def main():
         sys.exit(2)
     if args.only == "mood":
         print(json.dumps(mood_rate(steps=args.mood_steps)))
+        return
+    if args.only == "reacting":
+        print(json.dumps(reacting_rate(n=args.reacting_n, steps=args.reacting_steps)))
         return
     if args.only == "chem":
         print(json.dumps(chemistry_rate(1 << 20)))
@@ -383,6 +560,8 @@ def main():
         out["chemistry"] = chemistry_rate(1 << 20)
     if not args.skip_mood:
         out["mood"] = mood_rate(steps=args.mood_steps)
+    if not args.skip_reacting:
+        out["reacting"] = reacting_rate(n=args.reacting_n, steps=args.reacting_steps)
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

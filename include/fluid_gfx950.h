@@ -326,6 +326,29 @@ int fsg_chem_react(const double* table, double* U, int64_t ncells, int64_t nodes
                    const double* dt_elem, double dt, const fsg_chem_params* params,
                    void* stream);
 
+/* fsg_chem_integrate on the cells with flag[c] != 0 only: rho_out and T_out
+ * of the other cells are left as they are (the surrogate's predictions).
+ * list (n int32) and count (1 int32) are caller scratch; *count ends as the
+ * number of flagged cells.  No host synchronisation.  Same per-cell result as
+ * fsg_chem_integrate (integrator.py:9-104). */
+int fsg_chem_integrate_flagged(const double* table, const double* rho_in, const double* T_in,
+                               const double* dt_cell, double dt, int32_t newton_iters,
+                               int32_t constant_temperature, const int32_t* flag, int32_t* list,
+                               int32_t* count, double* rho_out, double* T_out, int64_t n,
+                               void* stream);
+
+/* The coupler's steps around its integrator call, for another substep in
+ * between (the surrogate): prepare = species clip and rescale to rho, thermal
+ * energy, T0 recovery (hybrid/chemistry/coupling.py:116-154) giving the
+ * integrator's rho_s (n, 11, kg/m^3) and T0 (n, K); finish = convert back,
+ * clip, rescale to rho and write the species (:165-177).  U is (n, 16) of
+ * dtype FSG_F64 or FSG_F32; chemistry in FP64. */
+int fsg_chem_prepare(const double* table, const void* U, int32_t dtype, int64_t ncells,
+                     const fsg_chem_params* params, double* rho_out, double* T_out,
+                     void* stream);
+int fsg_chem_finish(void* U, int32_t dtype, int64_t ncells, const fsg_chem_params* params,
+                    const double* rho_new, void* stream);
+
 const char* fsg_chem_last_error(void);
 
 /* ---- Chemistry surrogate (MLP 14 -> 128 -> 128 -> 128 -> 11) --------------

@@ -393,6 +393,113 @@ __global__ __launch_bounds__(64) void react_kernel(const double* __restrict__ ta
   for (int s = 0; s < NS; ++s) u[5 + s] = rs[s] * f;
 }
 
+// The coupler's steps before and after its integrator call
+// (hybrid/chemistry/coupling.py:116-154 and :165-177), split out so another
+// substep (the learned surrogate) can run in between.  Flow states of either
+// precision; chemistry in FP64.
+template <typename ureal>
+__global__ __launch_bounds__(64) void prepare_kernel(const double* __restrict__ tab,
+                                                     const ureal* __restrict__ U, int64_t n,
+                                                     fsg_chem_params p, double* __restrict__ rho_out,
+                                                     double* __restrict__ T_out) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const ureal* u = U + c * (5 + NS);
+  const double rt = dmax((double)u[0], 1.0e-14);
+  double rs[NS], sum = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rs[s] = dmax((double)u[5 + s], 0.0);
+    sum += rs[s];
+  }
+  const double scale = rt / dmax(sum, 1.0e-20);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rs[s] *= scale;
+  const double m0 = u[1], m1 = u[2], m2 = u[3];
+  const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / (rt * rt);
+  double T0;
+  if (p.dimensionless) {
+    const double eth_s = dmax((double)u[4] / rt - ek, 1.0e-4);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) rs[s] *= p.rho_ref;
+    const double Tg = dmax((p.gamma - 1.0) * eth_s * p.T_ref, 200.0);
+    T0 = recover_temperature(tab, rs, eth_s * p.e_ref, Tg, 8);
+  } else {
+    const double eth = dmax((double)u[4] / rt - ek, 1.0e3);
+    T0 = recover_temperature(tab, rs, eth, 1000.0, 8);
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rho_out[c * NS + s] = rs[s];
+  T_out[c] = T0;
+}
+
+template <typename ureal>
+__global__ __launch_bounds__(64) void finish_kernel(ureal* __restrict__ U, int64_t n,
+                                                    fsg_chem_params p,
+                                                    const double* __restrict__ rho_new) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  ureal* u = U + c * (5 + NS);
+  const double rt = dmax((double)u[0], 1.0e-14);
+  double rs[NS], s2 = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double r = rho_new[c * NS + s];
+    if (p.dimensionless) r = r / p.rho_ref;
+    rs[s] = dmax(r, 0.0);
+    s2 += rs[s];
+  }
+  const double f = rt / dmax(s2, 1.0e-20);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) u[5 + s] = (ureal)(rs[s] * f);
+}
+
+// Integrator on the flagged cells of a batch only (the surrogate's
+// fallback), without a host round trip: flagged cell ids are compacted with
+// wavefront ballots into list[0..*count), then a launch of one thread per
+// batch cell integrates list entries; threads past *count return at once.  Cells are independent, so the list order (set by the atomics) does
+// not change any result.
+__global__ void zero_count_kernel(int32_t* count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0;
+}
+
+__global__ __launch_bounds__(256) void compact_flags_kernel(const int32_t* __restrict__ flag,
+                                                            int64_t n, int32_t* __restrict__ list,
+                                                            int32_t* __restrict__ count) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool f = c < n && flag[c] != 0;
+  const uint64_t mask = __ballot(f);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && mask) base = atomicAdd(count, (int)__popcll(mask));
+  base = __shfl(base, 0, 64);
+  if (f) {
+    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+    list[base + rank] = (int32_t)c;
+  }
+}
+
+__global__ __launch_bounds__(64) void integrate_list_kernel(
+    const double* __restrict__ tab, const double* __restrict__ rho_in,
+    const double* __restrict__ T_in, const double* __restrict__ dt_cell, double dt, int iters,
+    int const_T, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+    double* __restrict__ rho_out, double* __restrict__ T_out) {
+  const int64_t m = *count;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = list[i];
+    double r[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) r[s] = rho_in[c * NS + s];
+    const double h = dt_cell ? dt_cell[c] : dt;
+    const double T = integrate(tab, r, T_in[c], h, iters, const_T);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) rho_out[c * NS + s] = r[s];
+    if (T_out) T_out[c] = T;
+  }
+}
+
 }  // namespace fsgchem
 
 static thread_local std::string g_chem_error;
@@ -430,6 +537,59 @@ int fsg_chem_react(const double* table, double* U, int64_t ncells, int64_t nodes
   hipLaunchKernelGGL(fsgchem::react_kernel, dim3(grid), dim3(64), 0,
                      static_cast<hipStream_t>(stream), table, U, ncells, nodes_per_elem, dt_elem,
                      dt, *params);
+  return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
+}
+
+int fsg_chem_integrate_flagged(const double* table, const double* rho_in, const double* T_in,
+                               const double* dt_cell, double dt, int32_t newton_iters,
+                               int32_t constant_temperature, const int32_t* flag, int32_t* list,
+                               int32_t* count, double* rho_out, double* T_out, int64_t n,
+                               void* stream) {
+  if (!table || !rho_in || !T_in || !flag || !list || !count || !rho_out)
+    return chem_fail(-10, "null argument");
+  if (n <= 0) return 0;
+  if (n > INT32_MAX) return chem_fail(-12, "batch too large for int32 cell ids");
+  if (newton_iters < 0) return chem_fail(-11, "negative iteration count");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(fsgchem::zero_count_kernel, dim3(1), dim3(64), 0, s, count);
+  hipLaunchKernelGGL(fsgchem::compact_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                     0, s, flag, n, list, count);
+  // one thread per possible list entry: blocks past *count return at once
+  hipLaunchKernelGGL(fsgchem::integrate_list_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64),
+                     0, s, table,
+                     rho_in, T_in, dt_cell, dt, newton_iters, constant_temperature, list, count,
+                     rho_out, T_out);
+  return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
+}
+
+int fsg_chem_prepare(const double* table, const void* U, int32_t dtype, int64_t ncells,
+                     const fsg_chem_params* params, double* rho_out, double* T_out,
+                     void* stream) {
+  if (!table || !U || !params || !rho_out || !T_out) return chem_fail(-10, "null argument");
+  if (ncells <= 0) return 0;
+  unsigned grid = (unsigned)((ncells + 63) / 64);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == FSG_F64)
+    hipLaunchKernelGGL(fsgchem::prepare_kernel<double>, dim3(grid), dim3(64), 0, s, table,
+                       static_cast<const double*>(U), ncells, *params, rho_out, T_out);
+  else
+    hipLaunchKernelGGL(fsgchem::prepare_kernel<float>, dim3(grid), dim3(64), 0, s, table,
+                       static_cast<const float*>(U), ncells, *params, rho_out, T_out);
+  return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
+}
+
+int fsg_chem_finish(void* U, int32_t dtype, int64_t ncells, const fsg_chem_params* params,
+                    const double* rho_new, void* stream) {
+  if (!U || !params || !rho_new) return chem_fail(-10, "null argument");
+  if (ncells <= 0) return 0;
+  unsigned grid = (unsigned)((ncells + 63) / 64);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == FSG_F64)
+    hipLaunchKernelGGL(fsgchem::finish_kernel<double>, dim3(grid), dim3(64), 0, s,
+                       static_cast<double*>(U), ncells, *params, rho_new);
+  else
+    hipLaunchKernelGGL(fsgchem::finish_kernel<float>, dim3(grid), dim3(64), 0, s,
+                       static_cast<float*>(U), ncells, *params, rho_new);
   return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
 }
 

@@ -117,6 +117,13 @@ def lib():
         L.fsg_chem_react.argtypes = [vp, vp, i64, i64, vp, f64, ctypes.POINTER(ChemParams), vp]
         L.fsg_chem_react.restype = ctypes.c_int
         L.fsg_chem_last_error.restype = ctypes.c_char_p
+        L.fsg_chem_integrate_flagged.argtypes = [vp, vp, vp, vp, ctypes.c_double, i32, i32, vp,
+                                                 vp, vp, vp, vp, i64, vp]
+        L.fsg_chem_integrate_flagged.restype = ctypes.c_int
+        L.fsg_chem_prepare.argtypes = [vp, vp, i32, i64, ctypes.POINTER(ChemParams), vp, vp, vp]
+        L.fsg_chem_prepare.restype = ctypes.c_int
+        L.fsg_chem_finish.argtypes = [vp, i32, i64, ctypes.POINTER(ChemParams), vp, vp]
+        L.fsg_chem_finish.restype = ctypes.c_int
         pp = ctypes.POINTER(vp)
         L.fsg_mfma_selftest.argtypes = [i32, vp, vp, vp, vp]
         L.fsg_set_option.argtypes = [vp, i32, i32]

@@ -71,14 +71,26 @@ class PointImplicitIntegrator:
 
 
 class ReactingFlowCoupler:
-    """hybrid/chemistry/coupling.py:19-209."""
+    """hybrid/chemistry/coupling.py:19-209.
+
+    With ``surrogate`` (a trained ``ChemistrySurrogate``) the integrator
+    substep is replaced by the surrogate with its integrator fallback; the
+    species clip/rescale and temperature recovery around it are the
+    coupler's own (coupling.py:116-177).  The flow state may be FP64 or FP32;
+    chemistry always runs in FP64."""
 
     def __init__(self, mechanism=None, relaxation=None, max_newton_iters=12, tolerance=1.0e-7,
                  dimensionless=False, reference_temperature=300.0,
-                 reference_pressure=101325.0, reference_length=1.0, gamma=1.4):
+                 reference_pressure=101325.0, reference_length=1.0, gamma=1.4, surrogate=None):
         self.mechanism = Air11ParkMechanism() if mechanism is None else mechanism
         self.integrator = PointImplicitIntegrator(self.mechanism, relaxation, max_newton_iters,
                                                   tolerance)
+        self.surrogate = surrogate
+        self.last_fallback = None
+        # FP64 states without a surrogate run the whole cell update in one
+        # kernel; False sends them through prepare -> integrator -> finish
+        # (same arithmetic).
+        self.fused = True
         self.num_species = 11
         self.species_names = SPECIES_NAMES
         self.dimensionless = bool(dimensionless)
@@ -107,8 +119,10 @@ class ReactingFlowCoupler:
         if U_block.shape[-1] < 5 + self.num_species:
             raise ValueError(f"State has {U_block.shape[-1]} variables; expected at least "
                              f"{5 + self.num_species} for reacting mixture.")
-        if U_block.dtype != torch.float64:
-            raise ValueError("chemistry runs in FP64: pass a float64 state")
+        if U_block.shape[-1] != 5 + self.num_species:
+            raise ValueError("the device coupler takes exactly 5 + 11 variables")
+        if U_block.dtype not in (torch.float64, torch.float32):
+            raise ValueError("flow state must be float64 or float32")
         out = U_block.contiguous().clone()
         E, N = out.shape[0], out.shape[1]
         dte = None
@@ -118,9 +132,33 @@ class ReactingFlowCoupler:
         else:
             dts = float(dt)
         p = self._params()
-        nat.check_chem(nat.lib().fsg_chem_react(
-            nat.ptr(self.mechanism.table), nat.ptr(out), E * N, N, nat.ptr(dte), dts,
-            ctypes.byref(p), nat.stream_handle(out.device)))
+        st = nat.stream_handle(out.device)
+        if self.surrogate is None and out.dtype == torch.float64 and self.fused:
+            nat.check_chem(nat.lib().fsg_chem_react(
+                nat.ptr(self.mechanism.table), nat.ptr(out), E * N, N, nat.ptr(dte), dts,
+                ctypes.byref(p), st))
+            return out
+        # prepare -> substep -> finish (coupling.py:116-154, the substep, :165-177)
+        n = E * N
+        dtype = nat.F64 if out.dtype == torch.float64 else nat.F32
+        rho = torch.empty(n, 11, dtype=torch.float64, device=out.device)
+        T0 = torch.empty(n, dtype=torch.float64, device=out.device)
+        nat.check_chem(nat.lib().fsg_chem_prepare(nat.ptr(self.mechanism.table), nat.ptr(out),
+                                                  dtype, n, ctypes.byref(p), nat.ptr(rho),
+                                                  nat.ptr(T0), st))
+        scale = self.t_ref if self.dimensionless else 1.0
+        if dte is not None:
+            h = (dte * scale).repeat_interleave(N)
+        else:
+            h = dts * scale
+        if self.surrogate is None:
+            new, _, _ = self.integrator.step(rho, T0, h, constant_temperature=False)
+        else:
+            new, _, fb = self.surrogate.step(rho, T0, h)
+            self.last_fallback = fb
+        new = new.contiguous()
+        nat.check_chem(nat.lib().fsg_chem_finish(nat.ptr(out), dtype, n, ctypes.byref(p),
+                                                 nat.ptr(new), st))
         return out
 
     def advance(self, states, dt):

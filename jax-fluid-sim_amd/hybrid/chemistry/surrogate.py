@@ -274,6 +274,10 @@ class ChemistrySurrogate:
 
     def infer(self, rho, T, dt, blocks=2048):
         """Projected predictions and fallback flags (no integrator call)."""
+        out, flag = self._infer(rho, T, dt, blocks)
+        return out, flag.bool()
+
+    def _infer(self, rho, T, dt, blocks=2048):
         w, b, c = self._pack()
         rho = torch.as_tensor(rho, dtype=torch.float64, device=self.device).contiguous()
         T = torch.as_tensor(T, dtype=torch.float64, device=self.device).contiguous()
@@ -291,15 +295,26 @@ class ChemistrySurrogate:
                                               nat.ptr(T), nat.ptr(dtt), dts, nat.ptr(out),
                                               nat.ptr(flag), n, blocks,
                                               nat.stream_handle(self.device)))
-        return out, flag.bool()
+        return out, flag
 
     def step(self, rho, T, dt):
-        """Surrogate step with the plain integrator on flagged cells."""
-        rho = torch.as_tensor(rho, dtype=torch.float64, device=self.device)
-        T = torch.as_tensor(T, dtype=torch.float64, device=self.device)
-        out, fb = self.infer(rho, T, dt)
-        idx = torch.nonzero(fb).flatten()
-        if idx.numel():
-            d = dt[idx] if torch.is_tensor(dt) and dt.numel() > 1 else dt
-            out[idx] = self.reference_step(rho[idx].contiguous(), T[idx].contiguous(), d)
-        return out, T, fb
+        """Surrogate step with the plain integrator on flagged cells, all on
+        the device (flagged cells are compacted by the library, no host
+        synchronisation).  Returns (rho_new, T, fallback_mask)."""
+        rho = torch.as_tensor(rho, dtype=torch.float64, device=self.device).contiguous()
+        T = torch.as_tensor(T, dtype=torch.float64, device=self.device).contiguous()
+        out, flag = self._infer(rho, T, dt)
+        n = rho.shape[0]
+        dtt, dts = None, 0.0
+        if torch.is_tensor(dt) and dt.numel() > 1:
+            dtt = dt.to(self.device, torch.float64).reshape(n).contiguous()
+        else:
+            dts = float(dt)
+        work = torch.empty(n + 1, dtype=torch.int32, device=self.device)
+        integ = self.integrator
+        nat.check_chem(nat.lib().fsg_chem_integrate_flagged(
+            nat.ptr(integ.mechanism.table), nat.ptr(rho), nat.ptr(T), nat.ptr(dtt), dts,
+            integ.max_newton_iters, int(self.constant_temperature), nat.ptr(flag),
+            nat.ptr(work[1:]), nat.ptr(work[:1]), nat.ptr(out), None, n,
+            nat.stream_handle(self.device)))
+        return out, T, flag.bool()

@@ -332,3 +332,7 @@ class ReactingFlowCoupler:
             for i in range(U.shape[1]):
                 out[e, i] = self.react_cell(U[e, i], float(dts[e]) if np.ndim(dts) else dts)
         return out
+
+    def advance(self, states, dt):
+        """hybrid/chemistry/coupling.py:199-209."""
+        return {k: self.react_block(v, dt) for k, v in states.items()}

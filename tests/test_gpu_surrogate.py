@@ -146,6 +146,8 @@ def test_out_of_hull_cells_fall_back_to_the_integrator(integrator):
                           constant_temperature=sur.constant_temperature)[0]
     assert torch.equal(full[::7], ref)
     assert torch.equal(fb2.cpu(), torch.as_tensor(fb))
+    keep = ~fb2
+    assert torch.equal(full[keep], out[keep])
 
 
 def test_surrogate_ablation(integrator):
