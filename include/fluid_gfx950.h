@@ -362,10 +362,14 @@ int fsg_mlp_linear_forward(const float* X, const float* W, const float* b, float
                            int32_t M, int32_t N, int32_t K, int32_t silu_on, void* stream);
 /* Backward of the layer above given dH (M x N, overwritten with dZ):
  * dW = dZ^T X (N x K), db = column sums of dZ (N), dX = dZ W (M x K; may be
- * null for the first layer). */
+ * null for the first layer).  The weight and bias gradients are summed over
+ * batch chunks into work (fsg_mlp_backward_work_size(M, N, K) floats, caller
+ * scratch) and reduced in a fixed order, so results are deterministic.
+ * N <= 256. */
 int fsg_mlp_linear_backward(float* dH, const float* Z, const float* X, const float* W, float* dW,
-                            float* db, float* dX, int32_t M, int32_t N, int32_t K,
+                            float* db, float* dX, float* work, int32_t M, int32_t N, int32_t K,
                             int32_t silu_on, void* stream);
+int64_t fsg_mlp_backward_work_size(int32_t M, int32_t N, int32_t K);
 /* g = scale * (O - T) over n entries (scale = 2 / n for the mean squared error). */
 int fsg_mlp_mse_grad(const float* O, const float* T, float* g, int64_t n, float scale,
                      void* stream);

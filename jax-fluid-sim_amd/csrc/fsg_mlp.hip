@@ -39,25 +39,30 @@ __global__ __launch_bounds__(256) void gemm_kernel(const float* __restrict__ A, 
                                                    float* __restrict__ C, int ldc,
                                                    const float* __restrict__ bias,
                                                    float* __restrict__ H, int M, int N, int K,
-                                                   int silu_on) {
+                                                   int silu_on, int kchunk, int64_t c_zstride) {
   __shared__ float sA[TK][TM + 4];
   __shared__ float sB[TK][TN + 4];
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  // blockIdx.z takes the k range [kbeg, kend) and writes its partial product
+  // to C + z * c_zstride (split-k; one z and kchunk = K otherwise)
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = kbeg + kchunk < K ? kbeg + kchunk : K;
+  C += (int64_t)blockIdx.z * c_zstride;
   float acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += TK) {
+  for (int k0 = kbeg; k0 < kend; k0 += TK) {
     for (int e = threadIdx.x; e < TK * TM; e += 256) {
       int kk = e / TM, mm = e % TM;
       int m = m0 + mm, k = k0 + kk;
       float v = 0.f;
-      if (m < M && k < K) v = TA ? A[(int64_t)k * lda + m] : A[(int64_t)m * lda + k];
+      if (m < M && k < kend) v = TA ? A[(int64_t)k * lda + m] : A[(int64_t)m * lda + k];
       sA[kk][mm] = v;
     }
     for (int e = threadIdx.x; e < TK * TN; e += 256) {
       int kk = e / TN, nn = e % TN;
       int n = n0 + nn, k = k0 + kk;
       float v = 0.f;
-      if (n < N && k < K) v = TB ? B[(int64_t)n * ldb + k] : B[(int64_t)k * ldb + n];
+      if (n < N && k < kend) v = TB ? B[(int64_t)n * ldb + k] : B[(int64_t)k * ldb + n];
       sB[kk][nn] = v;
     }
     __syncthreads();
@@ -103,13 +108,37 @@ __global__ void dsilu_kernel(float* __restrict__ g, const float* __restrict__ Z,
   if (i < n) g[i] *= dsilu(Z[i]);
 }
 
-// db[n] = sum_m dZ[m, n]; one thread per column (N <= 128), M rows.
-__global__ void colsum_kernel(const float* __restrict__ g, float* __restrict__ db, int M, int N) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+// Partial column sums of dZ over row chunk z: part[z * N + n] = sum over
+// rows [z * chunk, min(M, (z + 1) * chunk)) of dZ[m, n]; 256 threads stride
+// the columns (N <= 256).
+__global__ __launch_bounds__(256) void colsum_partial_kernel(const float* __restrict__ g,
+                                                             float* __restrict__ part, int M,
+                                                             int N, int chunk) {
+  const int n = threadIdx.x;
   if (n >= N) return;
+  const int mbeg = blockIdx.x * chunk;
+  const int mend = mbeg + chunk < M ? mbeg + chunk : M;
   float s = 0.f;
-  for (int m = 0; m < M; ++m) s += g[(int64_t)m * N + n];
-  db[n] = s;
+  for (int m = mbeg; m < mend; ++m) s += g[(int64_t)m * N + n];
+  part[(int64_t)blockIdx.x * N + n] = s;
+}
+
+// out[i] = sum over z = 0..S-1 of part[z * n + i], in order of z (the
+// split-k reduction; deterministic).
+__global__ void reduce_splits_kernel(const float* __restrict__ part, float* __restrict__ out,
+                                     int64_t n, int S) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < S; ++z) s += part[(int64_t)z * n + i];
+  out[i] = s;
+}
+
+// Split of the batch dimension for the weight / bias gradients: chunks of
+// at least 256 rows, at most 64 chunks.
+inline int backward_splits(int M) {
+  int S = (M + 255) / 256;
+  return S < 1 ? 1 : (S > 64 ? 64 : S);
 }
 
 // dO = 2 (O - T) / (M N)
@@ -390,6 +419,10 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
         Yf[s] = Yp[s] * exp(A[0][s] * lam[0] + A[1][s] * lam[1] + A[2][s] * lam[2]);
         zf += Yf[s];
       }
+      // Residual of each row relative to the size of the terms it sums, before
+      // and after the step (a row whose species are all absent initially, such
+      // as the charge of neutral air, has cvec = 0 and is measured against the
+      // predicted traces).
       res = 0.0;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -397,7 +430,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
 #pragma unroll
         for (int s = 0; s < NOUT; ++s) {
           acc += A[i][s] * Yf[s];
-          scale += fabs(A[i][s]) * Y0[s];
+          scale += fabs(A[i][s]) * (Y0[s] + Yf[s]);
         }
         double rel = fabs(acc) / (scale + 1e-300);
         res = rel > res ? rel : res;
@@ -455,29 +488,47 @@ int fsg_mlp_linear_forward(const float* X, const float* W, const float* b, float
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   dim3 grid((N + fsgmlp::TN - 1) / fsgmlp::TN, (M + fsgmlp::TM - 1) / fsgmlp::TM);
   hipLaunchKernelGGL((fsgmlp::gemm_kernel<false, true, 1>), grid, dim3(256), 0,
-                     static_cast<hipStream_t>(stream), X, K, W, K, Z, N, b, H, M, N, K, silu_on);
+                     static_cast<hipStream_t>(stream), X, K, W, K, Z, N, b, H, M, N, K, silu_on,
+                     K, (int64_t)0);
   return launched();
 }
 
-int fsg_mlp_linear_backward(float* dH, const float* Z, const float* X, const float* W, float* dW,
-                            float* db, float* dX, int32_t M, int32_t N, int32_t K,
-                            int32_t silu_on, void* stream) {
-  if (!dH || !Z || !X || !W || !dW || !db) return mlp_fail(-10, "null argument");
+int64_t fsg_mlp_backward_work_size(int32_t M, int32_t N, int32_t K) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
+  return (int64_t)fsgmlp::backward_splits(M) * ((int64_t)N * K + N);
+}
+
+int fsg_mlp_linear_backward(float* dH, const float* Z, const float* X, const float* W, float* dW,
+                            float* db, float* dX, float* work, int32_t M, int32_t N, int32_t K,
+                            int32_t silu_on, void* stream) {
+  if (!dH || !Z || !X || !W || !dW || !db || !work) return mlp_fail(-10, "null argument");
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  if (N > 256) return mlp_fail(-12, "layer width above 256");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int64_t n = (int64_t)M * N;
   if (silu_on)
     hipLaunchKernelGGL(fsgmlp::dsilu_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        dH, Z, n);
-  // dW[N, K] = dZ^T X : A(m = out n, k = batch) = dZ[k * N + n] (transposed)
-  dim3 gw((K + fsgmlp::TN - 1) / fsgmlp::TN, (N + fsgmlp::TM - 1) / fsgmlp::TM);
-  hipLaunchKernelGGL((fsgmlp::gemm_kernel<true, false, 0>), gw, dim3(256), 0, s, dH, N, X, K, dW,
-                     K, nullptr, nullptr, N, K, M, 0);
-  hipLaunchKernelGGL(fsgmlp::colsum_kernel, dim3((N + 127) / 128), dim3(128), 0, s, dH, db, M, N);
+  // dW[N, K] = dZ^T X : A(m = out n, k = batch) = dZ[k * N + n] (transposed),
+  // split over S batch chunks into work, then reduced in chunk order
+  const int S = fsgmlp::backward_splits(M);
+  const int chunk = (M + S - 1) / S;
+  const int64_t nk = (int64_t)N * K;
+  float* part_w = work;
+  float* part_b = work + (int64_t)S * nk;
+  dim3 gw((K + fsgmlp::TN - 1) / fsgmlp::TN, (N + fsgmlp::TM - 1) / fsgmlp::TM, S);
+  hipLaunchKernelGGL((fsgmlp::gemm_kernel<true, false, 0>), gw, dim3(256), 0, s, dH, N, X, K,
+                     part_w, K, nullptr, nullptr, N, K, M, 0, chunk, nk);
+  hipLaunchKernelGGL(fsgmlp::reduce_splits_kernel, dim3((unsigned)((nk + 255) / 256)), dim3(256),
+                     0, s, part_w, dW, nk, S);
+  hipLaunchKernelGGL(fsgmlp::colsum_partial_kernel, dim3(S), dim3(256), 0, s, dH, part_b, M, N,
+                     chunk);
+  hipLaunchKernelGGL(fsgmlp::reduce_splits_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256),
+                     0, s, part_b, db, (int64_t)N, S);
   if (dX) {
     dim3 gx((K + fsgmlp::TN - 1) / fsgmlp::TN, (M + fsgmlp::TM - 1) / fsgmlp::TM);
     hipLaunchKernelGGL((fsgmlp::gemm_kernel<false, false, 0>), gx, dim3(256), 0, s, dH, N, W, K,
-                       dX, K, nullptr, nullptr, M, K, N, 0);
+                       dX, K, nullptr, nullptr, M, K, N, 0, N, (int64_t)0);
   }
   return launched();
 }

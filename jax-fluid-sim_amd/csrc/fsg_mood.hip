@@ -49,6 +49,8 @@ struct TypeT {
   int N, Q, NS, NI, NB;
   real wsum;                   // sum of the reference volume weights
   real volsum;                 // sum of the reference subcell volumes
+  int KS;                      // ceil((N + NS) / 64): samples per lane
+  const real* mw;       // (N): mw[j] = sum_k wq[k] samples[j * NS + k], k < Q
   const real* p1proj;   // (N, N) transposed: P1[i][j] = p1proj[j * N + i]
   const real* samples;  // (N, NS): basis j at sample point k is samples[j * NS + k]
   const real* wq;
@@ -716,41 +718,44 @@ __global__ __launch_bounds__(256) void correct_kernel(Args<real> a, Phys<real> p
 
 // ---------------------------------------------------------------------------
 // Positivity (hybrid/limiter.py:69-147) of one element held in LDS, by one
-// wave; samples over lanes.  Writes the limited state back to sU.
-template <int V, typename real>
+// wave.  The volume mean is sum_j mw_j U_j / sum_k w_k (the quadrature of the
+// interpolant, with the basis-weight products summed once per type); lane l
+// holds the deviations of samples l, l + 64, ... (nodes, then the volume and
+// face sample points) in registers, KS of them.  Writes the limited state
+// back to sU.
+template <int V, int KS, typename real>
 __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const TypeT<real>& T,
-                                real (*sU)[V], real (*sD)[5], int lane) {
+                                real (*sU)[V], real* sM, int lane) {
   const int N = T.N, NT = N + T.NS;
-  // mean over the volume points
-  real msum[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) msum[v] = real(0);
-  for (int k = lane; k < T.Q; k += 64) {
-    real u[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) u[v] = real(0);
-    for (int j = 0; j < N; ++j) {
-      const real c = T.samples[(int64_t)j * T.NS + k];
-#pragma unroll
-      for (int v = 0; v < V; ++v) u[v] += c * sU[j][v];
-    }
-#pragma unroll
-    for (int v = 0; v < V; ++v) msum[v] += T.wq[k] * u[v];
+  if (lane < V) {
+    real m = real(0);
+    for (int j = 0; j < N; ++j) m += T.mw[j] * sU[j][lane];
+    sM[lane] = m / T.wsum;
   }
-  real mean[V];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  real mean[5];
 #pragma unroll
-  for (int v = 0; v < V; ++v) mean[v] = wsum(msum[v]) / T.wsum;
-  fsg::Prim<V, real> wm = fsg::c2p<V, real>(mean, ph.gamma);
+  for (int v = 0; v < 5; ++v) mean[v] = sM[v];
+  real mfull[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) mfull[v] = sM[v];
+  fsg::Prim<V, real> wm = fsg::c2p<V, real>(mfull, ph.gamma);
   const real rf = nmax(a.dfloor, a.rel * wm.rho);
   const real pf = nmax(a.pfloor, a.rel * wm.p);
   // sample deviations (first five variables) and the density minimum
+  real d[KS][5];
   real rmin = __builtin_huge_val();
-  for (int k = lane; k < NT; k += 64) {
+#pragma unroll
+  for (int i = 0; i < KS; ++i) {
+    const int k = lane + 64 * i;
     real u[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = mean[v];
     if (k < N) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) u[v] = sU[k][v];
-    } else {
+    } else if (k < NT) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) u[v] = real(0);
       for (int j = 0; j < N; ++j) {
@@ -759,23 +764,22 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
         for (int v = 0; v < 5; ++v) u[v] += c * sU[j][v];
       }
     }
-    rmin = nmin(rmin, u[0]);
+    if (k < NT) rmin = nmin(rmin, u[0]);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) sD[k][v] = u[v] - mean[v];
+    for (int v = 0; v < 5; ++v) d[i][v] = u[v] - mean[v];
   }
   rmin = wmin(rmin);
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const real den = mean[0] - rmin;
   real th = rmin < rf ? (mean[0] - rf) / (den > real(1e-30) ? den : real(1e-30)) : real(1);
   const real upper = th < real(0) ? real(0) : (th > real(1) ? real(1) : th);
   auto ok = [&](real t) {
     bool good = true;
-    for (int k = lane; k < NT; k += 64) {
+#pragma unroll
+    for (int i = 0; i < KS; ++i) {
       real u[5];
 #pragma unroll
-      for (int v = 0; v < 5; ++v) u[v] = mean[v] + t * sD[k][v];
-      good = good && (pressure(u, ph.gamma) >= pf);
+      for (int v = 0; v < 5; ++v) u[v] = mean[v] + t * d[i][v];
+      good = good && (lane + 64 * i >= NT || pressure(u, ph.gamma) >= pf);
     }
     return !wany(!good);
   };
@@ -791,23 +795,22 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
     }
     theta = lo;
   }
-  __builtin_amdgcn_wave_barrier();
   for (int k = lane; k < N * V; k += 64) {
     const int i = k / V, v = k % V;
-    sU[i][v] = mean[v] + theta * (sU[i][v] - mean[v]);
+    sU[i][v] = sM[v] + theta * (sU[i][v] - sM[v]);
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-constexpr int MAXNT = 64 + 640;   // nodes + sample points held in LDS
+constexpr int MAXNT = 64 + 640;   // nodes + sample points of one element
 
 // mode 0: out = positivity(src); mode 1 (stage): out = positivity(alpha base
 // + beta positivity(work)) with the node admissibility reductions.
-template <int V, typename real>
+template <int V, int KS, typename real>
 __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real> ph, int mode) {
   __shared__ real sU[MAXN][V];
-  __shared__ real sD[MAXNT][5];
+  __shared__ real sM[V];
   const int lane = threadIdx.x;
   int64_t e;
   const int b = locate(a, (int64_t)blockIdx.x, &e);
@@ -818,14 +821,14 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
   for (int k = lane; k < N * V; k += 64) sU[k / V][k % V] = src[e * N * V + k];
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  positivity_wave<V, real>(a, ph, T, sU, sD, lane);
+  positivity_wave<V, KS, real>(a, ph, T, sU, sM, lane);
   if (mode == 1) {
     const real* bs = a.base[b] + e * N * V;
     for (int k = lane; k < N * V; k += 64)
       sU[k / V][k % V] = a.alpha * bs[k] + a.beta * sU[k / V][k % V];
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    positivity_wave<V, real>(a, ph, T, sU, sD, lane);
+    positivity_wave<V, KS, real>(a, ph, T, sU, sM, lane);
   }
   real* out = a.out[b] + e * N * V;
   for (int k = lane; k < N * V; k += 64) out[k] = sU[k / V][k % V];
@@ -865,6 +868,8 @@ struct fsg_mood {
   void* d_types = nullptr;
   void* d_blocks = nullptr;
   void* d_groups = nullptr;
+  std::vector<void*> d_mw;      // per-type node weights of the volume mean
+  int ks = 1;                   // samples per lane of the positivity kernel
   std::vector<int64_t> offsets;
   std::vector<int32_t*> flags;
 };
@@ -920,6 +925,26 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
     for (real x : v) vs += x;
     d.wsum = ws;
     d.volsum = vs;
+    // mw[j] = sum_k w_k phi_j(x_k) over the volume points, summed in double
+    std::vector<real> smp((size_t)s.N * s.NS), mw(s.N);
+    if (s.Q > s.NS ||
+        hipMemcpy(smp.data(), s.samples, sizeof(real) * smp.size(), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+      return fsg_fail(-21, "cannot read the MOOD sample tables");
+    for (int j = 0; j < s.N; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < s.Q; ++k) acc += (double)w[k] * (double)smp[(size_t)j * s.NS + k];
+      mw[j] = (real)acc;
+    }
+    void* dmw = nullptr;
+    if (hipMalloc(&dmw, sizeof(real) * s.N) != hipSuccess)
+      return fsg_fail(-23, "cannot allocate the MOOD mean weights");
+    m->d_mw.push_back(dmw);
+    if (hipMemcpy(dmw, mw.data(), sizeof(real) * s.N, hipMemcpyHostToDevice) != hipSuccess)
+      return fsg_fail(-21, "cannot upload the MOOD mean weights");
+    d.mw = static_cast<const real*>(dmw);
+    d.KS = (s.N + s.NS + 63) / 64;
+    if (d.KS > m->ks) m->ks = d.KS;
   }
   const int nb = (int)c->blocks.size();
   std::vector<BlockT<real>> bb(nb);
@@ -1057,6 +1082,20 @@ int run_detect(fsg_mood* m, Args<real>& a, const Phys<real>& ph, hipStream_t s) 
 }
 
 template <int V, typename real>
+void launch_positivity(fsg_mood* m, const Args<real>& a, const Phys<real>& ph, int mode,
+                       hipStream_t s) {
+  const dim3 g((unsigned)m->total), b(64);
+  if (m->ks <= 4)
+    hipLaunchKernelGGL((positivity_kernel<V, 4, real>), g, b, 0, s, a, ph, mode);
+  else if (m->ks <= 5)
+    hipLaunchKernelGGL((positivity_kernel<V, 5, real>), g, b, 0, s, a, ph, mode);
+  else if (m->ks <= 8)
+    hipLaunchKernelGGL((positivity_kernel<V, 8, real>), g, b, 0, s, a, ph, mode);
+  else
+    hipLaunchKernelGGL((positivity_kernel<V, 11, real>), g, b, 0, s, a, ph, mode);
+}
+
+template <int V, typename real>
 int run_recompute_core(fsg_mood* m, Args<real>& a, const Phys<real>& ph, hipStream_t s) {
   const unsigned g4 = (unsigned)((m->total + 3) / 4);
   const unsigned gs = (unsigned)(m->total < SUBCELL_GRID ? m->total : SUBCELL_GRID);
@@ -1080,8 +1119,7 @@ int positivity_impl(fsg_mood* m, const void* const* U, void* const* out, hipStre
   set_ptrs(a.det_src, U, a.nblocks);
   set_ptrs(a.out, (const void* const*)out, a.nblocks);
   Phys<real> ph = phys_of<real>(m, nullptr);
-  hipLaunchKernelGGL((positivity_kernel<V, real>), dim3((unsigned)m->total), dim3(64), 0, s, a,
-                     ph, 0);
+  launch_positivity<V, real>(m, a, ph, 0, s);
   return launched("positivity");
 }
 
@@ -1120,8 +1158,7 @@ int recompute_impl(fsg_mood* m, const void* const* states, const void* const* ca
   }
   if ((rc = run_recompute_core<V, real>(m, a, ph, s))) return rc;
   for (int b = 0; b < nb; ++b) a.det_src[b] = a.work[b];
-  hipLaunchKernelGGL((positivity_kernel<V, real>), dim3((unsigned)m->total), dim3(64), 0, s, a,
-                     ph, 0);
+  launch_positivity<V, real>(m, a, ph, 0, s);
   return launched("positivity");
 }
 
@@ -1148,8 +1185,7 @@ int stage_impl(fsg_mood* m, const void* const* base, const void* const* curr,
   int rc;
   if ((rc = run_detect<V, real>(m, a, ph, s))) return rc;
   if ((rc = run_recompute_core<V, real>(m, a, ph, s))) return rc;
-  hipLaunchKernelGGL((positivity_kernel<V, real>), dim3((unsigned)m->total), dim3(64), 0, s, a,
-                     ph, 1);
+  launch_positivity<V, real>(m, a, ph, 1, s);
   return launched("positivity");
 }
 
@@ -1198,6 +1234,7 @@ void fsg_mood_destroy(fsg_mood* m) {
   (void)hipFree(m->d_types);
   (void)hipFree(m->d_blocks);
   (void)hipFree(m->d_groups);
+  for (void* p : m->d_mw) (void)hipFree(p);
   delete m;
 }
 

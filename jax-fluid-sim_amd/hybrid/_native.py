@@ -144,7 +144,10 @@ def lib():
             getattr(L, name).restype = ctypes.c_int
         f32 = ctypes.c_float
         L.fsg_mlp_linear_forward.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]
-        L.fsg_mlp_linear_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]
+        L.fsg_mlp_linear_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32,
+                                              vp]
+        L.fsg_mlp_backward_work_size.argtypes = [i32, i32, i32]
+        L.fsg_mlp_backward_work_size.restype = ctypes.c_int64
         L.fsg_mlp_mse_grad.argtypes = [vp, vp, vp, i64, f32, vp]
         L.fsg_mlp_adam.argtypes = [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]
         L.fsg_mlp_infer.argtypes = [vp, vp, vp, vp, vp, vp, f64, vp, vp, i64, i32, vp]

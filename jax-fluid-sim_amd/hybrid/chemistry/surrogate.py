@@ -204,9 +204,11 @@ class ChemistrySurrogate:
             dW = torch.empty_like(W)
             db = torch.empty_like(self.params[2 * k + 1])
             dX = torch.empty(M, K, device=self.device) if k > 0 else None
+            work = torch.empty(L.fsg_mlp_backward_work_size(M, N, K), device=self.device)
             nat.check_mlp(L.fsg_mlp_linear_backward(nat.ptr(g), nat.ptr(zs[k]), nat.ptr(acts[k]),
                                                     nat.ptr(W), nat.ptr(dW), nat.ptr(db),
-                                                    nat.ptr(dX), M, N, K, int(k < 3), st))
+                                                    nat.ptr(dX), nat.ptr(work), M, N, K,
+                                                    int(k < 3), st))
             grads[2 * k], grads[2 * k + 1] = dW, db
             g = dX
         loss = torch.mean((out - On) ** 2)

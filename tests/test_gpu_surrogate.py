@@ -8,6 +8,7 @@ import pytest
 import torch
 
 from hybrid.chemistry import Air11ParkMechanism, PointImplicitIntegrator
+from hybrid.chemistry.air11_park import CHARGES, MOLAR_MASSES
 from hybrid.chemistry.batch import make_batch
 from hybrid.chemistry.surrogate import (HIDDEN, N_IN, N_OUT, ChemistrySurrogate,
                                         log_ratio_targets, raw_features)
@@ -148,6 +149,30 @@ def test_out_of_hull_cells_fall_back_to_the_integrator(integrator):
     assert torch.equal(fb2.cpu(), torch.as_tensor(fb))
     keep = ~fb2
     assert torch.equal(full[keep], out[keep])
+
+
+def test_neutral_cells_pass_the_projection_check(integrator):
+    """Cells with no charged species (neutral air) have a zero charge row; the
+    projection's residual is measured against the predicted traces, so such
+    cells are not sent to the integrator (hull check switched off here)."""
+    sur, rho, T, _ = _fitted(integrator, n=2048, train_steps=200)
+    sur.stats["lo"] = torch.full_like(sur.stats["lo"], -1e30)
+    sur.stats["hi"] = torch.full_like(sur.stats["hi"], 1e30)
+    sur._packed = None
+    n = 256
+    Y = torch.zeros(n, 11, dtype=torch.float64, device="cuda")
+    Y[:, 9], Y[:, 10] = 0.767, 0.233
+    rt = torch.logspace(-3, -1, n, dtype=torch.float64, device="cuda")
+    Tn = torch.linspace(3000.0, 9000.0, n, dtype=torch.float64, device="cuda")
+    out, fb = sur.infer(rt[:, None] * Y, Tn, 1e-7)
+    assert not bool(fb.any())
+    # charge stays zero to round-off of the traces, mass is kept
+    q = (out[:, :6] * torch.as_tensor(CHARGES[:6], device="cuda", dtype=torch.float64)
+         / torch.as_tensor(MOLAR_MASSES[:6], device="cuda", dtype=torch.float64)).sum(1)
+    trace = (out[:, :6] / torch.as_tensor(MOLAR_MASSES[:6], device="cuda",
+                                          dtype=torch.float64)).sum(1)
+    assert bool((q.abs() <= 1e-9 * trace + 1e-300).all())
+    torch.testing.assert_close(out.sum(1), rt, rtol=1e-13, atol=0)
 
 
 def test_surrogate_ablation(integrator):
