@@ -550,7 +550,7 @@ def main():
         "fp32_dof_updates_per_s": res.get("f32", {}).get("rate"),
         "fp32_rel_l2_vs_fp64": None if err is None else
             dict(zip(["rho", "rho_u", "rho_v", "rho_w", "rho_E"], map(float, err))),
-        "roofline": {"kernel": f"element_kernel<hex,p3,V5,{key}>", "bound": "mfma",
+        "roofline": {"kernel": f"element_kernel<hex,p3,V5,{key}> (vector path)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK[key], "unit": "TFLOP/s",
                      "frac": achieved / PEAK[key], "traffic": traffic,
                      "flops_per_launch": flops, "avg_launch_ms": r["elem_ms"]},
