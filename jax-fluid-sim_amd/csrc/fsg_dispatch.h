@@ -6,7 +6,10 @@
 
 namespace fsg {
 
-constexpr int ELEM_THREADS = 128;
+// Threads per element in the vector element kernel: one wave when the
+// element has at most 64 volume points (hexahedra up to p = 2, where a
+// second wave would idle), else two (the points are the widest loop).
+constexpr int elem_threads(int T, int P) { return n_vol_q(T, P) <= 64 ? 64 : 128; }
 
 template <int P, int V, typename real>
 int launch_element(int type, int64_t count, const ElemArgs<real>& a, const Phys<real>& ph,
@@ -30,11 +33,13 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
       }                                                                                         \
       return hipGetLastError() == hipSuccess ? 0 : -1;                                          \
     }                                                                                           \
-    dim3 grid((unsigned)count), block(ELEM_THREADS);                                            \
+    constexpr int NTH = elem_threads(HEX, P), NTW = elem_threads(WEDGE, P);                    \
+    constexpr int NTT = elem_threads(TET, P);                                                   \
+    dim3 grid((unsigned)count);                                                                 \
     switch (type) {                                                                             \
-      case HEX: hipLaunchKernelGGL((element_kernel<HEX, P, V, real, ELEM_THREADS>), grid, block, 0, s, a, ph); break; \
-      case WEDGE: hipLaunchKernelGGL((element_kernel<WEDGE, P, V, real, ELEM_THREADS>), grid, block, 0, s, a, ph); break; \
-      case TET: hipLaunchKernelGGL((element_kernel<TET, P, V, real, ELEM_THREADS>), grid, block, 0, s, a, ph); break; \
+      case HEX: hipLaunchKernelGGL((element_kernel<HEX, P, V, real, NTH>), grid, dim3(NTH), 0, s, a, ph); break; \
+      case WEDGE: hipLaunchKernelGGL((element_kernel<WEDGE, P, V, real, NTW>), grid, dim3(NTW), 0, s, a, ph); break; \
+      case TET: hipLaunchKernelGGL((element_kernel<TET, P, V, real, NTT>), grid, dim3(NTT), 0, s, a, ph); break; \
       default: return -2;                                                                       \
     }                                                                                           \
     return hipGetLastError() == hipSuccess ? 0 : -1;                                            \

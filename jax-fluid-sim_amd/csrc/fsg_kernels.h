@@ -359,6 +359,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
 
   __shared__ real sUo[NO * V], sUn[NN * V], svo[NO * 4], svn[NN * 4];
   __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
+  __shared__ real sAo[QF * V], sAn[NBR ? QF * V : 1];
   __shared__ real sbo[FO * 12], sbn[FN * 12], sgo[QF * 12], sgn[QF * 12];
 
   const int tid = threadIdx.x;
@@ -394,52 +395,54 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   }
   __syncthreads();
 
-  // Phase 1: traces and the Euler numerical flux at each face point.
+  // Phase 1a: traces at the face points, one lane per (point, variable):
+  // primal Up = sum_k phi_k u_k (for the viscous flux) and anchored
+  // Ua = u_0 + sum_k phi_k (u_k - u_0) (for the Euler flux).
+  for (int k = tid; k < QF * V; k += 64) {
+    const int q = k / V, v = k - q * V;
+    const real* phio = tbo + q * FO;
+    const real u0 = sUo[v];
+    real ua = real(0), up = real(0);
+#pragma unroll
+    for (int kk = 0; kk < FO; ++kk) {
+      const real b = phio[kk];
+      const real u = sUo[supo[kk] * V + v];
+      up += b * u;
+      ua += b * (u - u0);
+    }
+    sAo[k] = ua + u0;
+    sPo[k] = up;
+    if constexpr (NBR) {
+      const real* phin = tbn + q * FN;
+      const real n0 = sUn[v];
+      real ub = real(0), vp = real(0);
+#pragma unroll
+      for (int kk = 0; kk < FN; ++kk) {
+        const real b = phin[kk];
+        const real u = sUn[supn[kk] * V + v];
+        vp += b * u;
+        ub += b * (u - n0);
+      }
+      sAn[k] = ub + n0;
+      sPn[k] = vp;
+    } else {
+      sPn[k] = up;  // unused: the boundary flux is the owner's
+    }
+  }
+  __syncthreads();
+
+  // Phase 1b: the Euler numerical flux at each face point.
   for (int q = tid; q < QF; q += 64) {
     const real w = a.fw[q] * area;
     const real* phio = tbo + q * FO;
-    real Ua_o[V], Up_o[V];
+    real Ua_o[V], Ua_n[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      Ua_o[v] = real(0);
-      Up_o[v] = real(0);
-    }
-#pragma unroll
-    for (int k = 0; k < FO; ++k) {
-      real b = phio[k];
-      const real* u = sUo + supo[k] * V;
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        Up_o[v] += b * u[v];
-        Ua_o[v] += b * (u[v] - sUo[v]);
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < V; ++v) Ua_o[v] += sUo[v];
-    real Ua_n[V], Up_n[V];
+    for (int v = 0; v < V; ++v) Ua_o[v] = sAo[q * V + v];
     if constexpr (NBR) {
-      const real* phin = tbn + q * FN;
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        Ua_n[v] = real(0);
-        Up_n[v] = real(0);
-      }
-#pragma unroll
-      for (int k = 0; k < FN; ++k) {
-        real b = phin[k];
-        const real* u = sUn + supn[k] * V;
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          Up_n[v] += b * u[v];
-          Ua_n[v] += b * (u[v] - sUn[v]);
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < V; ++v) Ua_n[v] += sUn[v];
+      for (int v = 0; v < V; ++v) Ua_n[v] = sAn[q * V + v];
     } else {
       ghost_state<V, real>(Ua_o, n, flags, ph, Ua_n);
-#pragma unroll
-      for (int v = 0; v < V; ++v) Up_n[v] = Up_o[v];  // unused: boundary flux is the owner's
     }
     real So[V], Sn[V];
 #pragma unroll
@@ -467,8 +470,6 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     for (int v = 0; v < V; ++v) {
       sSo[q * V + v] = So[v];
       sSn[q * V + v] = Sn[v];
-      sPo[q * V + v] = Up_o[v];
-      sPn[q * V + v] = Up_n[v];
     }
     if (visc) {
       real vo[4] = {0, 0, 0, 0}, vn[4] = {0, 0, 0, 0};
@@ -487,7 +488,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
           for (int m = 0; m < 4; ++m) vn[m] += b * svn[supn[k] * 4 + m];
         }
       } else {
-        ghost_viscous<V, real>(Up_o, n, flags, ph, vn);
+        ghost_viscous<V, real>(sPo + q * V, n, flags, ph, vn);
       }
 #pragma unroll
       for (int m = 0; m < 4; ++m) sJ[q * 4 + m] = w * (vn[m] - vo[m]);

@@ -743,27 +743,36 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
   fsg::Prim<V, real> wm = fsg::c2p<V, real>(mfull, ph.gamma);
   const real rf = nmax(a.dfloor, a.rel * wm.rho);
   const real pf = nmax(a.pfloor, a.rel * wm.p);
-  // sample deviations (first five variables) and the density minimum
+  // sample deviations (first five variables) and the density minimum; the
+  // node loop is outermost so each node's values are read from LDS once for
+  // all of a lane's samples (the sum over nodes keeps its order per sample)
   real d[KS][5];
+#pragma unroll
+  for (int i = 0; i < KS; ++i)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) d[i][v] = real(0);
+  for (int j = 0; j < N; ++j) {
+    real uj[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) uj[v] = sU[j][v];
+    const real* cj = T.samples + (int64_t)j * T.NS;
+#pragma unroll
+    for (int i = 0; i < KS; ++i) {
+      const int k = lane + 64 * i;
+      if (k >= N && k < NT) {
+        const real c = cj[k - N];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) d[i][v] += c * uj[v];
+      }
+    }
+  }
   real rmin = __builtin_huge_val();
 #pragma unroll
   for (int i = 0; i < KS; ++i) {
     const int k = lane + 64 * i;
     real u[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) u[v] = mean[v];
-    if (k < N) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) u[v] = sU[k][v];
-    } else if (k < NT) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) u[v] = real(0);
-      for (int j = 0; j < N; ++j) {
-        const real c = T.samples[(int64_t)j * T.NS + (k - N)];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) u[v] += c * sU[j][v];
-      }
-    }
+    for (int v = 0; v < 5; ++v) u[v] = k < N ? sU[k < N ? k : 0][v] : (k < NT ? d[i][v] : mean[v]);
     if (k < NT) rmin = nmin(rmin, u[0]);
 #pragma unroll
     for (int v = 0; v < 5; ++v) d[i][v] = u[v] - mean[v];
