@@ -356,6 +356,11 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   constexpr int TSN = QF * FN * 2 + QF * NN * 3;
   constexpr int W = V + 12;
   constexpr bool NBR = TN >= 0;
+  // Unroll depth of the trace and face-gradient loops: full, except in FP64
+  // with species, where full unrolling overflows the register file.
+  constexpr bool WIDE = V > 5 && sizeof(real) == 8;
+  constexpr int UO = WIDE ? 4 : FO, UN = WIDE ? 4 : FN;
+  constexpr int UG = WIDE ? 4 : NO, UGN = WIDE ? 4 : NN;
 
   __shared__ real sUo[NO * V], sUn[NN * V], svo[NO * 4], svn[NN * 4];
   __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
@@ -403,7 +408,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     const real* phio = tbo + q * FO;
     const real u0 = sUo[v];
     real ua = real(0), up = real(0);
-#pragma unroll
+#pragma unroll UO
     for (int kk = 0; kk < FO; ++kk) {
       const real b = phio[kk];
       const real u = sUo[supo[kk] * V + v];
@@ -416,7 +421,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
       const real* phin = tbn + q * FN;
       const real n0 = sUn[v];
       real ub = real(0), vp = real(0);
-#pragma unroll
+#pragma unroll UN
       for (int kk = 0; kk < FN; ++kk) {
         const real b = phin[kk];
         const real u = sUn[supn[kk] * V + v];
@@ -431,7 +436,91 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   }
   __syncthreads();
 
-  // Phase 1b: the Euler numerical flux at each face point.
+  // Phase 1b: the Euler numerical flux at each face point.  With species
+  // (V > 5) the flow columns stay in registers and each species column is
+  // streamed through LDS (numflux_split), which keeps the per-lane state to
+  // the five flow variables.
+  if constexpr (V > 5) {
+    for (int q = tid; q < QF; q += 64) {
+      const real w = a.fw[q] * area;
+      const real* Ao = sAo + q * V;
+      real UL[5], UR[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) UL[v] = Ao[v];
+      // ghost species (ghost_state): rho_g * Y, Y = far (inlet) or U_s / safe rho
+      real rho_g = real(0), safe_in = real(1);
+      if constexpr (NBR) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) UR[v] = sAn[q * V + v];
+      } else {
+        ghost_state<5, real>(UL, n, flags, ph, UR);
+        rho_g = UR[0];
+        safe_in = rmax(UL[0], real(1.0e-14));
+      }
+      auto sl = [&](int sp) { return Ao[sp]; };
+      auto sr = [&](int sp) -> real {
+        if constexpr (NBR) {
+          return sAn[q * V + sp];
+        } else {
+          if (!ph.has_boundary) return Ao[sp];
+          const real Y = (flags & BND_INLET) ? ph.farfield[sp] : Ao[sp] / safe_in;
+          return rho_g * Y;
+        }
+      };
+      if (euler) {
+        real F[5], Bo[5];
+        FluxScalars<real> fs;
+        numflux_split<V, real>(UL, UR, n, gamma, ph.rusanov, sl, sr, F, fs);
+        Prim<5, real> wa = c2p<5, real>(sUo, gamma);
+        normal_flux<5, real>(sUo, wa, n, Bo);
+        const real una = wa.vel[0] * n[0] + wa.vel[1] * n[1] + wa.vel[2] * n[2];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sSo[q * V + v] = -w * (F[v] - Bo[v]);
+        for (int sp = 5; sp < V; ++sp)
+          sSo[q * V + sp] = -w * (species_flux(fs, sl(sp), sr(sp)) - sUo[sp] * una);
+        if constexpr (NBR) {
+          real Bn[5];
+          Prim<5, real> wb = c2p<5, real>(sUn, gamma);
+          normal_flux<5, real>(sUn, wb, n, Bn);
+          const real unb = wb.vel[0] * n[0] + wb.vel[1] * n[1] + wb.vel[2] * n[2];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sSn[q * V + v] = w * (F[v] - Bn[v]);
+          for (int sp = 5; sp < V; ++sp)
+            sSn[q * V + sp] = w * (species_flux(fs, sl(sp), sr(sp)) - sUn[sp] * unb);
+        } else {
+          for (int v = 0; v < V; ++v) sSn[q * V + v] = real(0);
+        }
+      } else {
+        for (int v = 0; v < V; ++v) {
+          sSo[q * V + v] = real(0);
+          sSn[q * V + v] = real(0);
+        }
+      }
+      if (visc) {
+        const real* phio = tbo + q * FO;
+        real vo[4] = {0, 0, 0, 0}, vn[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < FO; ++k) {
+          real b = phio[k];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) vo[m] += b * svo[supo[k] * 4 + m];
+        }
+        if constexpr (NBR) {
+          const real* phin = tbn + q * FN;
+#pragma unroll
+          for (int k = 0; k < FN; ++k) {
+            real b = phin[k];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) vn[m] += b * svn[supn[k] * 4 + m];
+          }
+        } else {
+          ghost_viscous<V, real>(sPo + q * V, n, flags, ph, vn);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) sJ[q * 4 + m] = w * (vn[m] - vo[m]);
+      }
+    }
+  } else
   for (int q = tid; q < QF; q += 64) {
     const real w = a.fw[q] * area;
     const real* phio = tbo + q * FO;
@@ -608,6 +697,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
         int q = k >> 2, m = k & 3;
         const real* dph = tbo + QF * FO * 2 + q * NO * 3;
         real G0 = 0, G1 = 0, G2 = 0;
+#pragma unroll UG
         for (int i = 0; i < NO; ++i) {
           real vm = svo[i * 4 + m];
           G0 += dph[i * 3 + 0] * vm;
@@ -635,6 +725,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
           int q = k >> 2, m = k & 3;
           const real* dph = tbn + QF * FN * 2 + q * NN * 3;
           real G0 = 0, G1 = 0, G2 = 0;
+#pragma unroll UGN
           for (int i = 0; i < NN; ++i) {
             real vm = svn[i * 4 + m];
             G0 += dph[i * 3 + 0] * vm;

@@ -226,6 +226,112 @@ __device__ __forceinline__ void hllc(const real* UL, const real* UR, const real*
   }
 }
 
+// The numerical flux of hllc / rusanov above with the species columns kept
+// out of registers (states with many passive species).  The flow part
+// (columns 0-4) is computed here; every species column follows from the
+// scalars in FluxScalars and that column's two states alone, by the same
+// expressions as the all-column functions, so results are bit-identical.
+// sl(s) / sr(s) return the left / right state of species column s (s >= 5).
+template <typename real>
+struct FluxScalars {
+  int mode;     // 0: F_L, 1: left star, 2: right star, 3: F_R, 4: Rusanov
+  real unl, unr, S, comp, smax;
+};
+
+template <typename real>
+__device__ __forceinline__ real species_flux(const FluxScalars<real>& f, real uL, real uR) {
+#pragma clang fp contract(off)
+  const real FL = uL * f.unl, FR = uR * f.unr;
+  switch (f.mode) {
+    case 0: return FL;
+    case 1: return FL + f.S * (uL * f.comp - uL);
+    case 2: return FR + f.S * (uR * f.comp - uR);
+    case 3: return FR;
+    default: return real(0.5) * (FL + FR) - real(0.5) * f.smax * (uR - uL);
+  }
+}
+
+template <int V, typename real>
+__device__ __forceinline__ void rusanov_scalars(const real* UL, const real* UR, const real* n,
+                                                real gamma, FluxScalars<real>& f) {
+#pragma clang fp contract(off)
+  Prim<5, real> wl = c2p<5, real>(UL, gamma), wr = c2p<5, real>(UR, gamma);
+  real rl = rmax(wl.rho, real(1.0e-14)), rr = rmax(wr.rho, real(1.0e-14));
+  real pl = rmax(wl.p, real(0)), pr = rmax(wr.p, real(0));
+  real cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
+  f.mode = 4;
+  f.smax = rmax(fabs(f.unl) + cl, fabs(f.unr) + cr);
+}
+
+template <int V, typename real, class SL, class SR>
+__device__ __forceinline__ void numflux_split(const real* UL, const real* UR, const real* n,
+                                              real gamma, bool use_rusanov, SL sl, SR sr,
+                                              real* F, FluxScalars<real>& f) {
+#pragma clang fp contract(off)
+  Prim<5, real> wl = c2p<5, real>(UL, gamma), wr = c2p<5, real>(UR, gamma);
+  f.unl = wl.vel[0] * n[0] + wl.vel[1] * n[1] + wl.vel[2] * n[2];
+  f.unr = wr.vel[0] * n[0] + wr.vel[1] * n[1] + wr.vel[2] * n[2];
+  f.S = real(0);
+  f.comp = real(0);
+  f.smax = real(0);
+  if (use_rusanov) {
+    rusanov<5, real>(UL, UR, n, gamma, F);
+    rusanov_scalars<V, real>(UL, UR, n, gamma, f);
+    return;
+  }
+  real FL[5];
+  normal_flux<5, real>(UL, wl, n, FL);
+  bool same = true;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) same = same && (UL[v] == UR[v]);
+  for (int s = 5; s < V; ++s) same = same && (sl(s) == sr(s));
+  f.mode = 0;
+  if (same) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = FL[v];
+    return;
+  }
+  real FR[5];
+  normal_flux<5, real>(UR, wr, n, FR);
+  real cl = sqrt(rmax(gamma * wl.p / rmax(wl.rho, real(1.0e-14)), real(0)));
+  real cr = sqrt(rmax(gamma * wr.p / rmax(wr.rho, real(1.0e-14)), real(0)));
+  real SLs = rmin(f.unl - cl, f.unr - cr);
+  real SRs = rmax(f.unl + cl, f.unr + cr);
+  real den = wl.rho * (SLs - f.unl) - wr.rho * (SRs - f.unr);
+  real SM = (wr.p - wl.p + wl.rho * f.unl * (SLs - f.unl) - wr.rho * f.unr * (SRs - f.unr)) /
+            safe_signed(den);
+  real H[5];
+  if (SLs >= real(0)) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) H[v] = FL[v];
+  } else if (SM >= real(0)) {
+    hllc_side<5, real>(UL, wl, n, SLs, SM, FL, H);
+    f.mode = 1;
+    f.S = SLs;
+    f.comp = (SLs - f.unl) / safe_signed(SLs - SM);
+  } else if (SRs > real(0)) {
+    hllc_side<5, real>(UR, wr, n, SRs, SM, FR, H);
+    f.mode = 2;
+    f.S = SRs;
+    f.comp = (SRs - f.unr) / safe_signed(SRs - SM);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) H[v] = FR[v];
+    f.mode = 3;
+  }
+  bool ok = (wl.rho > real(0)) && (wr.rho > real(0)) && (wl.p > real(0)) && (wr.p > real(0));
+#pragma unroll
+  for (int v = 0; v < 5; ++v) ok = ok && finite(H[v]);
+  for (int s = 5; s < V; ++s) ok = ok && finite(species_flux(f, sl(s), sr(s)));
+  if (ok) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = H[v];
+  } else {
+    rusanov<5, real>(UL, UR, n, gamma, F);
+    rusanov_scalars<V, real>(UL, UR, n, gamma, f);
+  }
+}
+
 // hybrid/solver.py:307-315: [u, v, w, T = p / (rho R)].
 template <int V, typename real>
 __device__ __forceinline__ void viscous_variables(const real* U, real gamma, real R, real* v) {
