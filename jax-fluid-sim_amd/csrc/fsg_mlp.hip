@@ -135,9 +135,9 @@ __global__ void reduce_splits_kernel(const float* __restrict__ part, float* __re
 }
 
 // Split of the batch dimension for the weight / bias gradients: chunks of
-// at least 256 rows, at most 64 chunks.
+// at least 64 rows, at most 64 chunks.
 inline int backward_splits(int M) {
-  int S = (M + 255) / 256;
+  int S = (M + 63) / 64;
   return S < 1 ? 1 : (S > 64 ? 64 : S);
 }
 
