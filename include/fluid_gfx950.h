@@ -76,6 +76,8 @@ typedef struct {
   const int32_t* rec_sup; /* (R) offsets into the support-index array */
   const int32_t* rec_nf;  /* (R) support size */
   const void* minvT; /* (N, N) transposed reference inverse mass */
+  const void* sf1d;  /* hexahedra at p >= 2: 1-D factors B1 (Q1, N1), D1 (Q1, N1),
+                        M1^-1 (N1, N1) of the volume operators, or null */
 } fsg_block_desc;
 
 /* One face group: (face type, owner block, neighbour block) as in
@@ -173,6 +175,11 @@ int fsg_timing_read(fsg_context* ctx, double* group_ms, double* block_ms,
  * units (the faster of the two on MI355X for the p = 3 hexahedron in both
  * precisions); both compute the same discrete operator. */
 #define FSG_OPT_ELEMENT_MFMA 1
+/* FSG_OPT_HEX_SUM_FACTORISATION: 1 or -1 (default) evaluates the hexahedral
+ * volume operators at p >= 2 by sum factorisation (their 1-D factors, when
+ * the block carries sf1d), 0 by the dense operators; same discrete operator,
+ * round-off apart. */
+#define FSG_OPT_HEX_SUM_FACTORISATION 2
 int fsg_set_option(fsg_context* ctx, int32_t option, int32_t value);
 
 void fsg_destroy(fsg_context* ctx);

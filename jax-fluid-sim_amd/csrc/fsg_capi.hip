@@ -149,6 +149,7 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.wq = static_cast<const real*>(d.wq);
     a.minv = static_cast<const real*>(d.minv);
     a.minvT = static_cast<const real*>(d.minvT);
+    a.sf1d = c->hex_sum_fact != 0 ? static_cast<const real*>(d.sf1d) : nullptr;
     a.jinv = static_cast<const real*>(d.jinv);
     a.scale = static_cast<const real*>(d.scale);
     a.detw = static_cast<const real*>(d.detw);
@@ -268,6 +269,10 @@ int fsg_set_option(fsg_context* ctx, int32_t option, int32_t value) {
   if (!ctx) return fail(-10, "null argument");
   if (option == FSG_OPT_ELEMENT_MFMA) {
     ctx->element_mfma = value < 0 ? -1 : (value != 0);
+    return 0;
+  }
+  if (option == FSG_OPT_HEX_SUM_FACTORISATION) {
+    ctx->hex_sum_fact = value < 0 ? -1 : (value != 0);
     return 0;
   }
   return fail(-13, "unknown option %d", option);

@@ -19,7 +19,8 @@ struct fsg_context {
   std::vector<hipEvent_t> pool;
   std::vector<int> kinds;
   size_t used = 0;
-  int element_mfma = -1;  // FSG_OPT_ELEMENT_MFMA; -1: default (vector units)
+  int element_mfma = -1;
+  int hex_sum_fact = -1;  // FSG_OPT_HEX_SUM_FACTORISATION; -1: default (on)  // FSG_OPT_ELEMENT_MFMA; -1: default (vector units)
 };
 
 // Records a formatted error for fsg_last_error() and returns code.

@@ -3,6 +3,7 @@
 #pragma once
 #include "fsg_kernels.h"
 #include "fsg_elem_mfma.h"
+#include "fsg_hex_sf.h"
 
 namespace fsg {
 
@@ -32,6 +33,14 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
         default: return -2;                                                                     \
       }                                                                                         \
       return hipGetLastError() == hipSuccess ? 0 : -1;                                          \
+    }                                                                                           \
+    if constexpr (P >= 2) {                                                                     \
+      if (type == HEX && a.sf1d) {                                                              \
+        constexpr int NTS = P == 3 ? 256 : 64;                                                  \
+        hipLaunchKernelGGL((hex_sf_kernel<P, V, real, NTS>), dim3((unsigned)count), dim3(NTS), 0, \
+                           s, a, ph);                                                           \
+        return hipGetLastError() == hipSuccess ? 0 : -1;                                        \
+      }                                                                                         \
     }                                                                                           \
     constexpr int NTH = elem_threads(HEX, P), NTW = elem_threads(WEDGE, P);                    \
     constexpr int NTT = elem_threads(TET, P);                                                   \

@@ -40,6 +40,7 @@ struct ElemArgs {
   const real* wq;         // (Q)
   const real* minv;       // (N, N) reference inverse mass
   const real* minvT;      // (N, N) its transpose
+  const real* sf1d;       // hexahedra p >= 2: B1, D1, M1^-1 (or null: dense operators)
   const real* jinv;       // (E, 9)  [r*3+d] = d xi_r / d x_d
   const real* scale;      // (E) min(1, V/1e-4) / |det J|
   const real* detw;       // (E) |det J|

@@ -29,7 +29,8 @@ class BlockDesc(ctypes.Structure):
     _fields_ = [("element_type", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("count", ctypes.c_int64)] + [
         (k, ctypes.c_void_p) for k in ("Bt", "Dt", "D", "wq", "minv", "jinv", "scale", "detw",
-                                       "rec_ptr", "rec_off", "rec_sup", "rec_nf", "minvT")]
+                                       "rec_ptr", "rec_off", "rec_sup", "rec_nf", "minvT",
+                                       "sf1d")]
 
 
 class GroupDesc(ctypes.Structure):

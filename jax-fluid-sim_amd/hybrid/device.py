@@ -26,6 +26,30 @@ from .elements import LOCAL_FACES, TYPE_ID, basis, basis_gradients, corners, fac
     reference_tables
 from .topology import SLIVER_VOLUME, Topology
 
+
+def hex_factors(element_type, p, R):
+    """1-D factors [B1 (Q1, N1), D1 (Q1, N1), M1^-1 (N1, N1)] of the
+    hexahedral volume operators at p >= 2, flattened, or None.  The nodes and
+    the Gauss points are tensor products with x fastest (hybrid/elements.py),
+    so B = B1 (x) B1 (x) B1, d/dxi_x = B1 (x) B1 (x) D1 (cyclic) and
+    M^-1 = M1^-1 (x) M1^-1 (x) M1^-1; checked here against the dense tables."""
+    if element_type != "hexahedron" or p < 2:
+        return None
+    from .elements import _dline, _gauss01, _line
+    g, w = _gauss01(5 if p == 3 else 3)
+    B1, D1 = _line(p, g), _dline(p, g)
+    M1inv = np.linalg.inv(B1.T @ (w[:, None] * B1))
+    kron3 = lambda z, y, x: np.kron(z, np.kron(y, x))
+    checks = [(kron3(B1, B1, B1), R["B"]), (kron3(B1, B1, D1), R["D"][..., 0]),
+              (kron3(B1, D1, B1), R["D"][..., 1]), (kron3(D1, B1, B1), R["D"][..., 2])]
+    for approx, dense in checks:
+        if np.abs(approx - dense).max() > 1e-12 * max(1.0, np.abs(dense).max()):
+            raise RuntimeError("hexahedral tables are not tensor products in the expected order")
+    Mk = kron3(M1inv, M1inv, M1inv)
+    if np.abs(Mk - R["Minv"]).max() > 1e-9 * np.abs(R["Minv"]).max():
+        raise RuntimeError("hexahedral inverse mass is not the Kronecker product of its factors")
+    return np.concatenate([B1.ravel(), D1.ravel(), M1inv.ravel()])
+
 FACE_TYPE_NAME = ("quad", "triangle")
 
 
@@ -76,6 +100,8 @@ class DeviceMesh:
                 minv=self._t(R["Minv"]), minvT=self._t(R["Minv"].T),
                 jinv=self._t(b.jinv.reshape(-1, 9)),
                 scale=self._t(scale), detw=self._t(np.abs(b.det)), volume=vol)
+            sf = hex_factors(b.element_type, self.p, R)
+            t["sf1d"] = None if sf is None else self._t(sf)
             self.block_tables.append(t)
 
     # -- face groups -------------------------------------------------------
@@ -200,6 +226,7 @@ class DeviceMesh:
             for k in ("Bt", "Dt", "D", "wq", "minv", "minvT", "jinv", "scale", "detw", "rec_ptr",
                       "rec_off", "rec_sup", "rec_nf"):
                 setattr(d, k, t[k].data_ptr())
+            d.sf1d = 0 if t["sf1d"] is None else t["sf1d"].data_ptr()
         groups = (nat.GroupDesc * max(len(self.groups), 1))()
         for i, g in enumerate(self.groups):
             d = groups[i]
@@ -234,6 +261,12 @@ class DeviceMesh:
                 self.ctx = None
         except Exception:
             pass
+
+    def set_hex_sum_factorisation(self, on=None):
+        """Hexahedral volume operators at p >= 2 by their 1-D factors (True
+        or None: the default) or dense (False); round-off apart, the same
+        operator."""
+        nat.check(nat.lib().fsg_set_option(self.ctx, 2, -1 if on is None else int(bool(on))))
 
     def set_element_path(self, mfma=None):
         """Element operators as batched matrix-core products (True) or one

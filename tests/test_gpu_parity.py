@@ -193,10 +193,11 @@ def test_in_place_stage_matches_out_of_place():
 @pytest.mark.parametrize("name,p", [("smoke", 2), ("split", 3), ("tgv", 3), ("species", 2)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_both_element_paths(name, p, dtype):
-    """The batched matrix-core element path and the one-workgroup-per-element
-    vector-unit path compute the same operator: both match the oracle, in
-    FP64 at the FP64 bar and in FP32 at the FP32 bar."""
-    print(f"case {name} p={p} {dtype} both element paths", flush=True)
+    """The element paths compute the same operator: batched matrix-core
+    products, one workgroup per element with the dense reference operators,
+    and (hexahedra at p >= 2) sum factorisation over the 1-D factors.  Each
+    matches the oracle, in FP64 at the FP64 bar and in FP32 at the FP32 bar."""
+    print(f"case {name} p={p} {dtype} all element paths", flush=True)
     ds, dop, db, oop, ob, states = _case(name, p, dtype=dtype)
     U = ds.to_device(states)
     ref = oop.rhs(states, ob)
@@ -205,11 +206,15 @@ def test_both_element_paths(name, p, dtype):
     else:
         bar = FP32_TOL.get(name, 5e-5) if name != "smoke" else 5e-5
     out = {}
+    dm = dop.device_mesh
     try:
-        for mfma in (True, False):
-            dop.device_mesh.set_element_path(mfma=mfma)
-            out[mfma] = _np(dop.rhs(U, db))
+        for path, mfma, sf in (("mfma", True, False), ("dense", False, False),
+                               ("sum-factorised", False, True)):
+            dm.set_element_path(mfma=mfma)
+            dm.set_hex_sum_factorisation(sf)
+            out[path] = _np(dop.rhs(U, db))
     finally:
-        dop.device_mesh.set_element_path(None)
-    for mfma, got in out.items():
-        assert np.all(rel_l2(got, ref) <= bar), (mfma, rel_l2(got, ref), bar)
+        dm.set_element_path(None)
+        dm.set_hex_sum_factorisation(None)
+    for path, got in out.items():
+        assert np.all(rel_l2(got, ref) <= bar), (path, rel_l2(got, ref), bar)
