@@ -11,8 +11,9 @@ stages) on state already resident in HBM.  The plain FP64 path and the FP32
 path run the same number of steps from the same initial state with the same
 dt; ``value`` is the DOF-updates/s per RK stage of the FP32 path when its
 relative L2 error against the FP64 path stays within 1e-4 for density and
-energy, else of the FP64 path.  ``roofline`` reports the element kernel (the
-dominant launch) timed with HIP events on its own stream; ``cpu_baseline``
+energy, else of the FP64 path.  ``roofline`` reports the face kernel (the
+longest launch, bound by HBM latency and bandwidth) timed with HIP events on
+its own stream, ``element_kernel`` the sum-factorised element kernel; ``cpu_baseline``
 times the CPU FP64 oracle on a bounded sample (8^3 elements, 2 stages).
 """
 
@@ -49,6 +50,28 @@ def element_flops(N, Q, V):
     2N*3Q*V, inverse mass 2N^2 V."""
     return (2 * Q * N * V + 2 * Q * N * 12 + 2 * Q * 36 + 2 * N * N * 12 + 2 * Q * N * 12
             + 2 * N * 3 * Q * V + 2 * N * N * V)
+
+
+def hex_sf_flops(P, V):
+    """Algorithmic flops of one hexahedron in the sum-factorised element
+    kernel (viscous terms on): the 1-D passes at 2 flops per multiply-add --
+    lift M^-1 (3 passes x 12 columns), interpolation x / y / z passes of the
+    state, lift and viscous-variable columns, the three transposed divergence
+    passes, M^-1 on the residual -- plus 370 flops per volume point for the
+    pointwise Euler and viscous fluxes and the pull-back."""
+    N1, Q1 = P + 1, (5 if P == 3 else 3)
+    N, Q, CI = N1 ** 3, Q1 ** 3, V + 16
+    return (3 * N * 12 * N1 * 2 + N1 * N1 * (CI + 4) * Q1 * N1 * 2
+            + N1 * Q1 * (CI + 8) * Q1 * N1 * 2 + Q1 * Q1 * (V + 24) * Q1 * N1 * 2
+            + Q1 * Q1 * 3 * V * N1 * Q1 * 2 + N1 * Q1 * V * N1 * Q1 * 6
+            + N1 * N1 * V * N1 * Q1 * 4 + 3 * N * V * N1 * 2 + 370 * Q)
+
+
+def face_bytes(N, NF, V, itemsize):
+    """Algorithmic HBM bytes of one interior face in the face kernel: both
+    elements' nodal states read, both sides' records (V surface values and
+    12 lift values per support node) written."""
+    return (2 * N * V + 2 * NF * (V + 12)) * itemsize
 
 
 def measured_traffic(kernel, key):
@@ -533,9 +556,13 @@ def main():
     within = err is not None and err[0] <= 1e-4 and err[4] <= 1e-4
     key = "f32" if within else ("f64" if "f64" in res else "f32")
     r = res[key]
-    flops = element_flops(N, Q, V) * E
-    achieved = flops / (r["elem_ms"] * 1e-3) / 1e12
-    traffic = measured_traffic("element", key) if args.n == 32 else None
+    nfaces = 3 * E
+    itemsize = 4 if key == "f32" else 8
+    fbytes = face_bytes(N, 16, V, itemsize) * nfaces
+    face_gbs = fbytes / (r["face_ms"] * 1e-3) / 1e9
+    eflops = hex_sf_flops(3, V) * E
+    elem_tf = eflops / (r["elem_ms"] * 1e-3) / 1e12
+    traffic = measured_traffic("face", key) if args.n == 32 else None
     out = {
         "metric": "DOF-updates/s per RK stage, plain FP64 vs FP32+surrogate chem, at fixed L2 error",
         "value": r["rate"], "unit": "DOF-updates/s per RK stage", "n_gpus": 1,
@@ -550,11 +577,16 @@ def main():
         "fp32_dof_updates_per_s": res.get("f32", {}).get("rate"),
         "fp32_rel_l2_vs_fp64": None if err is None else
             dict(zip(["rho", "rho_u", "rho_v", "rho_w", "rho_E"], map(float, err))),
-        "roofline": {"kernel": f"element_kernel<hex,p3,V5,{key}> (vector path)", "bound": "mfma",
-                     "achieved": achieved, "peak": PEAK[key], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[key], "traffic": traffic,
-                     "flops_per_launch": flops, "avg_launch_ms": r["elem_ms"]},
-        "face_kernel_avg_ms": r["face_ms"],
+        "roofline": {"kernel": f"face_kernel<quad,hex,hex,p3,V5,{key}>", "bound": "hbm",
+                     "achieved": face_gbs, "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": face_gbs / HBM_PEAK, "traffic": traffic,
+                     "bytes_per_launch": fbytes, "faces": nfaces, "avg_launch_ms": r["face_ms"]},
+        "element_kernel": {"kernel": f"hex_sf_kernel<p3,V5,{key}> (sum factorisation)",
+                           "avg_launch_ms": r["elem_ms"], "flops_per_launch": eflops,
+                           "achieved_tflops": elem_tf, "peak_tflops": PEAK[key],
+                           "frac": elem_tf / PEAK[key],
+                           "traffic": measured_traffic("element", key) if args.n == 32 else None,
+                           "dense_operator_flops_per_launch": element_flops(N, Q, V) * E},
     }
     if not args.skip_chem:
         out["chemistry"] = chemistry_rate(1 << 20)
