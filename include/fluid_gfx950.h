@@ -105,6 +105,14 @@ typedef struct {
   const int32_t* bflags;    /* (F) FSG_BND_* flags (boundary groups) */
   int64_t out_owner;        /* offset (reals) of the owner records in face_out */
   int64_t out_neighbor;
+  /* Hexahedral quad sides at p >= 2 (or null): per trace table 4 + 2 Qf
+   * int32 [normal axis, face plane 0/1, tangential axes t1 < t2, the point's
+   * index along t1 (Qf), along t2 (Qf)], and the 1-D factors [B1 (Qf1, N1),
+   * D1 (Qf1, N1), l(0), l'(0), l(1), l'(1) (N1 each)] at the Qf1 face Gauss
+   * points, for the BR2 face gradients by sum factorisation. */
+  const int32_t* owner_sf;
+  const int32_t* neighbor_sf;
+  const void* face_sf1d;
 } fsg_group_desc;
 
 /* Boundary object (FarfieldWallBoundary, hybrid/boundary.py:6-102).

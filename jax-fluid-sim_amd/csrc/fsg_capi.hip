@@ -105,6 +105,11 @@ int run_faces(fsg_context* c, const void* const* U, int terms, const Phys<real>&
     a.owner_sup = g.owner_sup;
     a.fw = static_cast<const real*>(g.face_weights);
     a.bflags = g.bflags;
+    if (c->hex_sum_fact != 0 && g.face_sf1d) {
+      a.owner_sf = g.owner_sf;
+      a.neighbor_sf = g.neighbor_sf;
+      a.face_sf1d = static_cast<const real*>(g.face_sf1d);
+    }
     a.jinv_o = static_cast<const real*>(bo.jinv);
     a.scale_o = static_cast<const real*>(bo.scale);
     a.out_o = fo + g.out_owner;

@@ -72,6 +72,9 @@ struct FaceArgs {
   const int32_t* neighbor_sup;  // (tables, FN)
   const real* fw;          // (Qf) normalised face weights x reference measure
   const int32_t* bflags;   // (F) boundary flags
+  const int32_t* owner_sf;    // hexahedral quad sides: per-table sum-factorisation map, or null
+  const int32_t* neighbor_sf;
+  const real* face_sf1d;      // 1-D factors at the face Gauss points
   const real* jinv_o;
   const real* scale_o;
   const real* jinv_n;
@@ -346,6 +349,96 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   }
 }
 
+// BR2 face gradient of one hexahedral side on a quad face (p >= 2) by sum
+// factorisation: the face points are the Qf1 x Qf1 Gauss grid on one face
+// plane of the element, so the raw reference gradient
+//   d v / d xi_r (q) = sum_n d phi_n / d xi_r (x_q) v_n
+// is a contraction along the face normal (l(plane) or l'(plane)), then along
+// the two tangential axes (B1 or D1 at the face Gauss points).  meta holds
+// [normal axis, plane 0/1, t1, t2, index along t1 (Qf), index along t2 (Qf)]
+// for this trace table.  Writes sg[q * 12 + m * 3 + d] = J^-T grad + eta
+// scale (local lift), as the dense phase does.  scratch: 2 N1^2 * 4 +
+// 3 N1 Qf1 * 4 reals of LDS.
+template <int P, typename real>
+__device__ __forceinline__ void face_grad_hex_sf(const int32_t* __restrict__ meta,
+                                                 const real* __restrict__ sf1, const real* sv,
+                                                 const real* lop_base, const real* sbb,
+                                                 const real* Jx, real sc, real eta, real* sg,
+                                                 real* scratch, int tid) {
+  constexpr int N1 = P + 1, QF1 = P == 2 ? 3 : 4, QF = QF1 * QF1, NF = N1 * N1;
+  const int dn = meta[0], plane = meta[1], t1 = meta[2], t2 = meta[3];
+  const int32_t* aq = meta + 4;
+  const int32_t* bq = meta + 4 + QF;
+  const real* Lg = sf1;
+  const real* dLg = sf1 + QF1 * N1;
+  const real* en = sf1 + 2 * QF1 * N1 + plane * 2 * N1;
+  const real* den = en + N1;
+  auto stride = [](int axis) { return axis == 0 ? 1 : (axis == 1 ? N1 : N1 * N1); };
+  const int st1 = stride(t1), st2 = stride(t2), stn = stride(dn);
+  real* Vn = scratch;                  // [i1][i2][m]
+  real* Dn = Vn + NF * 4;
+  real* A = Dn + NF * 4;               // [i1][b][m]
+  real* B = A + N1 * QF1 * 4;
+  real* C = B + N1 * QF1 * 4;
+  for (int t = tid; t < NF * 4; t += 64) {          // normal contraction
+    const int m = t & 3, i12 = t >> 2, i1 = i12 / N1, i2 = i12 - i1 * N1;
+    const int base = i1 * st1 + i2 * st2;
+    real vn = real(0), dv = real(0);
+#pragma unroll
+    for (int k = 0; k < N1; ++k) {
+      const real x = sv[(base + k * stn) * 4 + m];
+      vn += en[k] * x;
+      dv += den[k] * x;
+    }
+    Vn[t] = vn;
+    Dn[t] = dv;
+  }
+  __syncthreads();
+  for (int t = tid; t < N1 * QF1 * 4; t += 64) {    // along t2
+    const int m = t & 3, r = t >> 2, i1 = r / QF1, b = r - i1 * QF1;
+    real av = real(0), bv = real(0), cv = real(0);
+#pragma unroll
+    for (int i2 = 0; i2 < N1; ++i2) {
+      const int src = ((i1 * N1 + i2) << 2) + m;
+      const real l = Lg[b * N1 + i2], dl = dLg[b * N1 + i2];
+      av += l * Vn[src];
+      bv += dl * Vn[src];
+      cv += l * Dn[src];
+    }
+    A[t] = av;
+    B[t] = bv;
+    C[t] = cv;
+  }
+  __syncthreads();
+  for (int t = tid; t < QF * 4; t += 64) {          // along t1, per (point, variable)
+    const int q = t >> 2, m = t & 3, qa = aq[q], qb = bq[q];
+    real g1 = real(0), g2 = real(0), gn = real(0);
+#pragma unroll
+    for (int i1 = 0; i1 < N1; ++i1) {
+      const int src = ((i1 * QF1 + qb) << 2) + m;
+      g1 += dLg[qa * N1 + i1] * A[src];
+      g2 += Lg[qa * N1 + i1] * B[src];
+      gn += Lg[qa * N1 + i1] * C[src];
+    }
+    real G[3];
+    G[t1] = g1;
+    G[t2] = g2;
+    G[dn] = gn;
+    const real* lop = lop_base + q * NF;
+    real L[3] = {real(0), real(0), real(0)};
+#pragma unroll
+    for (int kk = 0; kk < NF; ++kk) {
+      const real c = lop[kk];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) L[d] += c * sbb[kk * 12 + m * 3 + d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      sg[q * 12 + m * 3 + d] =
+          G[0] * Jx[0 * 3 + d] + G[1] * Jx[1 * 3 + d] + G[2] * Jx[2 * 3 + d] + eta * (sc * L[d]);
+  }
+}
+
 template <int FT, int TO, int TN, int SUB, int P, int V, typename real>
 __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> ph) {
   constexpr int NO = n_nodes(TO, P);
@@ -361,11 +454,20 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   // with species, where full unrolling overflows the register file.
   constexpr bool WIDE = V > 5 && sizeof(real) == 8;
   constexpr int UO = WIDE ? 4 : FO, UN = WIDE ? 4 : FN;
-  constexpr int UG = WIDE ? 4 : NO, UGN = WIDE ? 4 : NN;
+  // the dense face-gradient loops are only a fallback on hexahedral quad
+  // sides (sum factorisation below): keep them short there
+  constexpr bool HQ = FT == QUAD && SUB == 0 && P >= 2;
+  constexpr int UG = (WIDE || (HQ && TO == HEX)) ? 4 : NO;
+  constexpr int UGN = (WIDE || (HQ && TN == HEX)) ? 4 : NN;
 
   __shared__ real sUo[NO * V], sUn[NN * V], svo[NO * 4], svn[NN * 4];
   __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
   __shared__ real sAo[QF * V], sAn[NBR ? QF * V : 1];
+  // BR2 face gradients by sum factorisation on hexahedral quad sides
+  constexpr bool SFO = FT == QUAD && SUB == 0 && TO == HEX && P >= 2;
+  constexpr bool SFN = FT == QUAD && SUB == 0 && TN == HEX && P >= 2;
+  constexpr int SFN1 = P + 1, SFQ1 = P == 2 ? 3 : 4;
+  __shared__ real sfs[(SFO || SFN) ? 2 * SFN1 * SFN1 * 4 + 3 * SFN1 * SFQ1 * 4 : 1];
   __shared__ real sbo[FO * 12], sbn[FN * 12], sgo[QF * 12], sgn[QF * 12];
 
   const int tid = threadIdx.x;
@@ -629,9 +731,38 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     __syncthreads();
 
     // Phase 3: face gradients = raw polynomial gradient + eta * local lift.
-    // FP32: four lanes per face point split the element nodes and the support
-    // nodes (the gradient table is read once per (point, node)); FP64 keeps one
-    // lane per (point, variable), which needs fewer registers per lane.
+    // Hexahedral quad sides (p >= 2) by sum factorisation when the tables are
+    // given; otherwise the dense gradient tables.
+    // FP32 dense: four lanes per face point split the element nodes and the
+    // support nodes (the gradient table is read once per (point, node)); FP64
+    // dense keeps one lane per (point, variable), which needs fewer registers.
+    bool sf_o = false, sf_n = false;
+    if constexpr (SFO || SFN) {
+      if (a.face_sf1d) {
+        constexpr int MW = 4 + 2 * QF;
+        if constexpr (SFO) {
+          if (a.owner_sf) {
+          real Jo[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
+          face_grad_hex_sf<P, real>(a.owner_sf + to_id * MW, a.face_sf1d, svo, tbo + QF * FO, sbo,
+                                    Jo, a.scale_o[eo], ph.eta, sgo, sfs, tid);
+          sf_o = true;
+          __syncthreads();
+          }
+        }
+        if constexpr (SFN) {
+          if (a.neighbor_sf) {
+          real Jn[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
+          face_grad_hex_sf<P, real>(a.neighbor_sf + a.neighbor_table[f] * MW, a.face_sf1d, svn,
+                                    tbn + QF * FN, sbn, Jn, a.scale_n[en], ph.eta, sgn, sfs, tid);
+          sf_n = true;
+          }
+        }
+      }
+    }
     if constexpr (sizeof(real) == 4) {
       static_assert(QF <= 16 || QF == 25, "face points per wave");
       auto side = [&](const real* tb, const real* sv, const real* sbb, const real* Jx, real sc,
@@ -679,22 +810,26 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
                                          G[m * 3 + 2] * Jx[2 * 3 + d] + ph.eta * (sc * L[m * 3 + d]);
         }
       };
-      real Jo[9];
+      if (!sf_o) {
+        real Jo[9];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
-      side(tbo, svo, sbo, Jo, a.scale_o[eo], NO, FO, sgo);
+        for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
+        side(tbo, svo, sbo, Jo, a.scale_o[eo], NO, FO, sgo);
+      }
       if constexpr (NBR) {
-        real Jn[9];
+        if (!sf_n) {
+          real Jn[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
-        side(tbn, svn, sbn, Jn, a.scale_n[en], NN, FN, sgn);
+          for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
+          side(tbn, svn, sbn, Jn, a.scale_n[en], NN, FN, sgn);
+        }
       }
     } else {
       const real so = a.scale_o[eo];
       real Jo[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
-      for (int k = tid; k < QF * 4; k += 64) {
+      for (int k = sf_o ? QF * 4 : tid; k < QF * 4; k += 64) {
         int q = k >> 2, m = k & 3;
         const real* dph = tbo + QF * FO * 2 + q * NO * 3;
         real G0 = 0, G1 = 0, G2 = 0;
@@ -722,7 +857,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
         real Jn[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
-        for (int k = tid; k < QF * 4; k += 64) {
+        for (int k = sf_n ? QF * 4 : tid; k < QF * 4; k += 64) {
           int q = k >> 2, m = k & 3;
           const real* dph = tbn + QF * FN * 2 + q * NN * 3;
           real G0 = 0, G1 = 0, G2 = 0;

@@ -40,7 +40,8 @@ class GroupDesc(ctypes.Structure):
         (k, ctypes.c_void_p) for k in ("owner_elem", "neighbor_elem", "geom", "owner_table",
                                        "neighbor_table", "owner_tables", "neighbor_tables",
                                        "owner_sup", "neighbor_sup", "face_weights", "bflags")] + [
-        ("out_owner", ctypes.c_int64), ("out_neighbor", ctypes.c_int64)]
+        ("out_owner", ctypes.c_int64), ("out_neighbor", ctypes.c_int64)] + [
+        (k, ctypes.c_void_p) for k in ("owner_sf", "neighbor_sf", "face_sf1d")]
 
 
 class MoodType(ctypes.Structure):

@@ -27,6 +27,53 @@ from .elements import LOCAL_FACES, TYPE_ID, basis, basis_gradients, corners, fac
 from .topology import SLIVER_VOLUME, Topology
 
 
+def _face_gauss(p):
+    from .elements import _gauss01
+    return _gauss01({2: 3}.get(p, 4))[0]
+
+
+def hex_face_factors(p):
+    """1-D factors at the quad-face Gauss points for the sum-factorised BR2
+    face gradients of hexahedral sides: [B1 (Qf1, N1), D1 (Qf1, N1), l(0),
+    l'(0), l(1), l'(1)], flattened; None below p = 2."""
+    if p < 2:
+        return None
+    from .elements import _dline, _line
+    g = _face_gauss(p)
+    ends = [f(p, np.array([x]))[0] for x in (0.0, 1.0) for f in (_line, _dline)]
+    return np.concatenate([_line(p, g).ravel(), _dline(p, g).ravel(), *ends])
+
+
+def hex_face_map(element_type, p, pts):
+    """For a trace table of a hexahedral side on a quad face (p >= 2): the
+    face points are the Gauss grid on one face plane, so return [normal axis,
+    plane 0/1, tangential axes t1 < t2, each point's Gauss index along t1,
+    along t2] (int), else None."""
+    if element_type != "hexahedron" or p < 2:
+        return None
+    g = _face_gauss(p)
+    if len(pts) != len(g) ** 2:
+        return None
+    for dn in range(3):
+        col = pts[:, dn]
+        if np.all(np.abs(col - col[0]) < 1e-12) and min(abs(col[0]), abs(col[0] - 1)) < 1e-12:
+            break
+    else:
+        return None
+    t1, t2 = [d for d in range(3) if d != dn]
+    idx = []
+    for t in (t1, t2):
+        d = np.abs(pts[:, t][:, None] - g[None, :])
+        k = d.argmin(1)
+        if d[np.arange(len(pts)), k].max() > 1e-12:
+            return None
+        idx.append(k)
+    # the grid must be complete: every (a, b) pair exactly once
+    if len(set(zip(idx[0].tolist(), idx[1].tolist()))) != len(pts):
+        return None
+    return np.concatenate([[dn, int(round(pts[0, dn])), t1, t2], idx[0], idx[1]])
+
+
 def hex_factors(element_type, p, R):
     """1-D factors [B1 (Q1, N1), D1 (Q1, N1), M1^-1 (N1, N1)] of the
     hexahedral volume operators at p >= 2, flattened, or None.  The nodes and
@@ -112,7 +159,7 @@ class DeviceMesh:
         key = np.round(points.reshape(F, -1), 11)
         uniq, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
         R = reference_tables(element_type, p)
-        blobs, sups = [], []
+        blobs, sups, metas = [], [], []
         for u, f0 in enumerate(first):
             pts = points[f0]
             sup = _support(element_type, int(lfaces[f0]), p)
@@ -121,6 +168,9 @@ class DeviceMesh:
             lift = (phi @ R["Minv"])[:, sup]
             blobs.append(np.concatenate([phi[:, sup].ravel(), lift.ravel(), dphi.ravel()]))
             sups.append(sup)
+            metas.append(hex_face_map(element_type, p, pts))
+        meta = None if any(m is None for m in metas) else np.stack(metas).astype(np.int32)
+        self._last_meta = meta
         return np.stack(blobs), np.stack(sups).astype(np.int32), inv.ravel().astype(np.int32)
 
     def _build_groups(self):
@@ -142,6 +192,7 @@ class DeviceMesh:
             to = self.topo.blocks[ob].element_type
             opts = np.einsum("qc,fcr->fqr", shapes, f.owner_corners[sel, :nv])
             otab, osup, oidx = self._tables(to, f.owner_face[sel], opts)
+            omap = self._last_meta
             FO = n_face_nodes(ft, sub, p)
             assert osup.shape[1] == FO, (to, ft, sub, osup.shape)
             g = dict(face_type=int(ft), owner_block=int(ob), neighbor_block=int(nb),
@@ -150,6 +201,7 @@ class DeviceMesh:
             g["geom"] = self._t(np.concatenate([f.normal[sel], f.area[sel, None]], 1))
             g["owner_table"] = self._t(oidx, torch.int32)
             g["owner_tables"] = self._t(otab)
+            g["owner_sf"] = None if omap is None else self._t(omap, torch.int32)
             g["owner_sup_host"] = osup
             g["owner_sup_base"] = sup_base
             support_chunks.append(osup.ravel())
@@ -161,6 +213,8 @@ class DeviceMesh:
                 tn = self.topo.blocks[nb].element_type
                 npts = np.einsum("qc,fcr->fqr", shapes, f.neighbor_corners[sel, :nv])
                 ntab, nsup, nidx = self._tables(tn, f.neighbor_face[sel], npts)
+                g["neighbor_sf"] = (None if self._last_meta is None
+                                    else self._t(self._last_meta, torch.int32))
                 FN = n_face_nodes(ft, 0, p)
                 assert nsup.shape[1] == FN, (tn, ft, nsup.shape)
                 g["FN"] = FN
@@ -187,6 +241,8 @@ class DeviceMesh:
                 g["neighbor_sup"] = self.support[g["neighbor_sup_base"]:]
         self.face_out = torch.empty(max(offset, 1), dtype=self.dtype, device=self.device)
         self.keep.append(self.face_out)
+        ff = hex_face_factors(p)
+        self.face_sf1d = None if ff is None else self._t(ff)
 
     def _build_csr(self):
         W = self.nvars + 12
@@ -235,9 +291,10 @@ class DeviceMesh:
             d.count = g["count"]
             for k in ("owner_elem", "geom", "owner_table", "owner_tables", "owner_sup",
                       "face_weights", "bflags", "neighbor_elem", "neighbor_table",
-                      "neighbor_tables", "neighbor_sup"):
+                      "neighbor_tables", "neighbor_sup", "owner_sf", "neighbor_sf"):
                 v = g.get(k)
                 setattr(d, k, 0 if v is None else v.data_ptr())
+            d.face_sf1d = 0 if self.face_sf1d is None else self.face_sf1d.data_ptr()
             d.out_owner = g["out_owner"]
             d.out_neighbor = g["out_neighbor"]
         self._params = params
