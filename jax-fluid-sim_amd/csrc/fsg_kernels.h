@@ -88,7 +88,11 @@ struct FaceArgs {
 template <typename real>
 __device__ __forceinline__ void atomic_min_nan(double* addr, double v) {
   unsigned long long* p = reinterpret_cast<unsigned long long*>(addr);
-  unsigned long long old = *p, assumed;
+  // coherent read first: almost every caller finds the minimum already below
+  // its value and returns without a read-modify-write (a plain load may see a
+  // stale cached line and send every workgroup into the CAS loop)
+  unsigned long long old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long assumed;
   do {
     assumed = old;
     double cur = __longlong_as_double(static_cast<long long>(assumed));

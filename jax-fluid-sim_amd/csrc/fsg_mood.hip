@@ -50,6 +50,7 @@ struct TypeT {
   real wsum;                   // sum of the reference volume weights
   real volsum;                 // sum of the reference subcell volumes
   int KS;                      // ceil((N + NS) / 64): samples per lane
+  real lebesgue;               // max over samples k of sum_j |phi_j(x_k)| (>= 1)
   const real* mw;       // (N): mw[j] = sum_k wq[k] samples[j * NS + k], k < Q
   const real* p1proj;   // (N, N) transposed: P1[i][j] = p1proj[j * N + i]
   const real* samples;  // (N, NS): basis j at sample point k is samples[j * NS + k]
@@ -743,6 +744,48 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
   fsg::Prim<V, real> wm = fsg::c2p<V, real>(mfull, ph.gamma);
   const real rf = nmax(a.dfloor, a.rel * wm.rho);
   const real pf = nmax(a.pfloor, a.rel * wm.p);
+  // Shortcut: every sample is mean + sum_j phi_j(x_k) (U_j - mean) (the
+  // basis sums to one), so |u_k - mean| <= L max_j |U_j - mean| with L the
+  // type's Lebesgue constant over its samples.  When the resulting lower
+  // bounds on rho and p (less a generous allowance for the rounding of the
+  // sample sums and of the pressure) clear the floors, the full evaluation
+  // would find theta_rho = theta_p = 1; the limited state is then
+  // mean + 1 (U - mean), as below, without sampling.
+  {
+    real D[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const real dev = lane < N ? fabs(sU[lane][v] - mean[v]) : real(0);
+      D[v] = wmax(dev);
+    }
+    const real L = T.lebesgue;
+    const real eps = sizeof(real) == 4 ? real(1.2e-7) : real(2.3e-16);
+    const real rnd = real(16) * real(N + 2) * eps * L;
+    real b[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) b[v] = L * D[v] + rnd * (fabs(mean[v]) + D[v]);
+    const real rho_lb = mean[0] - b[0];
+    const real E_lb = mean[4] - b[4];
+    real m2 = real(0);
+#pragma unroll
+    for (int d = 1; d < 4; ++d) {
+      const real mu = fabs(mean[d]) + b[d];
+      m2 += mu * mu;
+    }
+    const real ke_ub = rho_lb > real(0) ? real(0.5) * m2 / rho_lb : real(0);
+    const real p_lb = (ph.gamma - real(1)) * (E_lb - ke_ub);
+    const real p_tol = real(16) * eps * (ph.gamma - real(1)) * (fabs(mean[4]) + b[4] + ke_ub);
+    const bool safe = rho_lb > real(0) && rho_lb >= rf && p_lb - p_tol >= pf;
+    if (safe) {
+      for (int k = lane; k < N * V; k += 64) {
+        const int i = k / V, v = k % V;
+        sU[i][v] = sM[v] + real(1) * (sU[i][v] - sM[v]);
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      return;
+    }
+  }
   // sample deviations (first five variables) and the density minimum; the
   // node loop is outermost so each node's values are read from LDS once for
   // all of a lane's samples (the sum over nodes keeps its order per sample)
@@ -945,6 +988,14 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
       for (int k = 0; k < s.Q; ++k) acc += (double)w[k] * (double)smp[(size_t)j * s.NS + k];
       mw[j] = (real)acc;
     }
+    // Lebesgue constant of the samples (nodes: 1), rounded up
+    double leb = 1.0;
+    for (int k = 0; k < s.NS; ++k) {
+      double acc = 0.0;
+      for (int j = 0; j < s.N; ++j) acc += fabs((double)smp[(size_t)j * s.NS + k]);
+      leb = acc > leb ? acc : leb;
+    }
+    d.lebesgue = (real)(leb * (1.0 + 1e-6));
     void* dmw = nullptr;
     if (hipMalloc(&dmw, sizeof(real) * s.N) != hipSuccess)
       return fsg_fail(-23, "cannot allocate the MOOD mean weights");
