@@ -107,6 +107,14 @@ __device__ __forceinline__ double nan_min(double a, double b) {
   return a < b ? a : b;
 }
 
+// Lanes per node in the element kernel's divergence: the largest power of
+// two DS <= 64 with N DS <= NT.
+__host__ __device__ constexpr int div_split(int N, int NT) {
+  int d = 1;
+  while (d < 64 && N * d * 2 <= NT) d *= 2;
+  return d;
+}
+
 template <int T, int P, int V, typename real, int NT>
 __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real> ph) {
   constexpr int N = n_nodes(T, P);
@@ -118,7 +126,6 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   __shared__ real sb[N * 12];
   __shared__ real sL[N * 12];
   __shared__ real sF[Q * 3 * V];
-  __shared__ real sP[2 * N * V];
   __shared__ double red[2][NT / 64];
   __shared__ unsigned int redn[NT / 64];
 
@@ -262,27 +269,32 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   }
   __syncthreads();
 
-  // R_i += sum_q sum_r dphi_i/dxi_r F^_r: two threads per node, each over
-  // half of the points and all variables (one read of D per point and
-  // direction), partial sums combined through LDS.
-  for (int k = tid; k < N * 2; k += NT) {
-    const int i = k >> 1, h = k & 1;
-    const int q0 = h ? Q / 2 : 0, q1 = h ? Q : Q / 2;
+  // R_i += sum_q sum_r dphi_i/dxi_r F^_r: DS adjacent lanes per node (DS a
+  // power of two with N DS <= NT: 2 for 64 nodes, 8 for the 10-node
+  // tetrahedron), each over every DS-th point and all variables (one read of
+  // D per point and direction), partial sums combined by lane shuffles.
+  {
+    constexpr int DS = div_split(N, NT);
+    const int i = tid / DS, h = tid - i * DS;
     real acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = real(0);
-    for (int q = q0; q < q1; ++q)
+    if (i < N)
+      for (int q = h; q < Q; q += DS)
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const real d = a.D[(q * 3 + r) * N + i];
+        for (int r = 0; r < 3; ++r) {
+          const real d = a.D[(q * 3 + r) * N + i];
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] += d * sF[(q * 3 + r) * V + v];
-      }
+          for (int v = 0; v < V; ++v) acc[v] += d * sF[(q * 3 + r) * V + v];
+        }
 #pragma unroll
-    for (int v = 0; v < V; ++v) sP[(h * N + i) * V + v] = acc[v];
+    for (int o = 1; o < DS; o <<= 1)
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], o, 64);
+    if (i < N && h == 0)
+#pragma unroll
+      for (int v = 0; v < V; ++v) sR[i * V + v] += acc[v];
   }
-  __syncthreads();
-  for (int k = tid; k < N * V; k += NT) sR[k] += sP[k] + sP[N * V + k];
   __syncthreads();
 
   real* out = a.out + e * (N * V);
@@ -291,27 +303,38 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
     return;
   }
   const real dte = a.dt_elem ? a.dt_elem[e] : a.dt;
-  // rhs = scale M^-1 R: one thread per node, all variables per row read.
-  for (int i = tid; i < N; i += NT) {
-    real acc[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) acc[v] = real(0);
-    for (int j = 0; j < N; ++j) {
-      const real m = a.minv[i * N + j];
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[v] += m * sR[j * V + v];
+  // rhs = scale M^-1 R.  With at least two threads per node, one thread per
+  // (node, variable); otherwise one thread per node over all variables (one
+  // read of each M^-1 row).
+  auto finish = [&](int k, real acc) {
+    const real rhs = s * acc;
+    if (a.mode == 1) {
+      out[k] = rhs;
+    } else {
+      const real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (sU[k] + dte * rhs);
+      out[k] = nxt;
+      sF[k] = nxt;
     }
+  };
+  if constexpr (2 * N <= NT) {
+    for (int k = tid; k < N * V; k += NT) {
+      const int i = k / V, v = k - i * V;
+      real acc = real(0);
+      for (int j = 0; j < N; ++j) acc += a.minv[i * N + j] * sR[j * V + v];
+      finish(k, acc);
+    }
+  } else {
+    for (int i = tid; i < N; i += NT) {
+      real acc[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const int k = i * V + v;
-      const real rhs = s * acc[v];
-      if (a.mode == 1) {
-        out[k] = rhs;
-      } else {
-        const real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (sU[k] + dte * rhs);
-        out[k] = nxt;
-        sF[k] = nxt;
+      for (int v = 0; v < V; ++v) acc[v] = real(0);
+      for (int j = 0; j < N; ++j) {
+        const real m = a.minv[i * N + j];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += m * sR[j * V + v];
       }
+#pragma unroll
+      for (int v = 0; v < V; ++v) finish(i * V + v, acc[v]);
     }
   }
   if (a.mode != 2) return;
