@@ -314,7 +314,37 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
       sO[wave][cellw][row] = out[j] + sb[3 * HID + row];
     }
     __syncthreads();
+    // The network output of each cell goes to its own rho_out slot as NOUT
+    // floats (the slot holds NOUT doubles); project_kernel reads them there.
     if (lane < 16 && valid) {
+      float* o = reinterpret_cast<float*>(rho_out + c * NOUT);
+#pragma unroll
+      for (int s = 0; s < NOUT; ++s) o[s] = sO[wave][lane][s];
+    }
+    __syncthreads();
+  }
+}
+
+// Log-ratio softmax, then the minimum-KL projection onto the element,
+// charge and mass constraints of the input (FP64), one cell per thread; the
+// network output is read from the cell's rho_out slot (NOUT floats written by
+// infer_kernel) before the slot is overwritten with the result.
+__global__ __launch_bounds__(64) void project_kernel(const InferConsts cstv,
+                                                     const double* __restrict__ rho_in,
+                                                     const double* __restrict__ T_in,
+                                                     const double* __restrict__ dt_cell,
+                                                     double dt, double* __restrict__ rho_out,
+                                                     int32_t* __restrict__ flag, int64_t n) {
+  const InferConsts* cst = &cstv;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  float onet[NOUT];
+  {
+    const float* o = reinterpret_cast<const float*>(rho_out + c * NOUT);
+#pragma unroll
+    for (int s = 0; s < NOUT; ++s) onet[s] = o[s];
+  }
+  {
       // Log-ratio softmax, then the minimum-KL projection onto the element,
       // charge and mass constraints of the input (FP64).
       double r[NOUT], rt = 0.0;
@@ -327,7 +357,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
 #pragma unroll
       for (int s = 0; s < NOUT; ++s) {
         Y0[s] = r[s] / rt;
-        double o = (double)sO[wave][lane][s] * cst->out_std[s] + cst->out_mean[s];
+        double o = (double)onet[s] * cst->out_std[s] + cst->out_mean[s];
         Yp[s] = log(dmax_(Y0[s], 0.0) + (double)cst->eps_y) + o;
         mx = Yp[s] > mx ? Yp[s] : mx;
       }
@@ -462,8 +492,6 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
       flag[c] = ok ? 0 : 1;
 #pragma unroll
       for (int s = 0; s < NOUT; ++s) rho_out[c * NOUT + s] = ok ? rt * Yf[s] : r[s];
-    }
-    __syncthreads();
   }
 }
 
@@ -571,6 +599,10 @@ int fsg_mlp_infer(const void* packed_weights, const float* biases, const void* c
                      static_cast<hipStream_t>(stream),
                      static_cast<const __bf16*>(packed_weights), biases, c, rho_in, T_in,
                      dt_cell, dt, rho_out, fallback, n);
+  if (int rc = launched()) return rc;
+  hipLaunchKernelGGL(fsgmlp::project_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), c, rho_in, T_in, dt_cell, dt, rho_out,
+                     fallback, n);
   return launched();
 }
 
