@@ -374,6 +374,46 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     const real r1 = hi1 / (sc1 > real(1e-12) ? sc1 : real(1e-12));
     ratio = nmax(r0, r1);
   }
+  // PAD shortcut, as in the positivity limiter: every sample is
+  // mean + sum_j phi_j(x_k) (U_j - mean), so with finite, moderate nodal
+  // values and the Lebesgue-constant bounds on rho and p at or above the
+  // floors (less a rounding allowance), no sample, and not the mean (a
+  // positive-weight average of the volume samples; p is concave), can fail:
+  // the sampled check below would find nothing.
+  bool pad_done = false;
+  {
+    real mpart = real(0);
+    if (lane < 5)
+      for (int j = 0; j < N; ++j) mpart += T.mw[j] * sU[w][j][lane];
+    real mv[5], D[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) mv[v] = __shfl(mpart, v, 64) / T.wsum;
+    const real big = sizeof(real) == 4 ? real(1e15) : real(1e150);
+    bool fine = true;
+    if (lane < N)
+#pragma unroll
+      for (int v = 0; v < V; ++v) fine = fine && isnum(sU[w][lane][v]) && fabs(sU[w][lane][v]) <= big;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) D[v] = wmax(lane < N ? fabs(sU[w][lane][v] - mv[v]) : real(0));
+    const real L = T.lebesgue;
+    const real eps = sizeof(real) == 4 ? real(1.2e-7) : real(2.3e-16);
+    const real rnd = real(16) * real(N + 2) * eps * L;
+    real bnd[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) bnd[v] = L * D[v] + rnd * (fabs(mv[v]) + D[v]);
+    const real rho_lb = mv[0] - bnd[0];
+    real m2 = real(0);
+#pragma unroll
+    for (int d = 1; d < 4; ++d) {
+      const real mu = fabs(mv[d]) + bnd[d];
+      m2 += mu * mu;
+    }
+    const real ke_ub = rho_lb > real(0) ? real(0.5) * m2 / rho_lb : real(0);
+    const real p_lb = (ph.gamma - real(1)) * (mv[4] - bnd[4] - ke_ub);
+    const real p_tol = real(16) * eps * (ph.gamma - real(1)) * (fabs(mv[4]) + bnd[4] + ke_ub);
+    pad_done = !wany(!fine) && rho_lb > real(0) && rho_lb >= a.dfloor &&
+               p_lb - p_tol >= a.pfloor;
+  }
   // PAD over nodes, volume points, face sample points and the mean.
   bool bad = false;
   real msum[V];
@@ -387,6 +427,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     for (int v = 0; v < V; ++v) fin = fin && isnum(W[v]);
     return !fin || W[0] < a.dfloor || W[4] < a.pfloor;
   };
+  if (!pad_done) {
   if (lane < N) bad = bad || check(sU[w][lane]);
   for (int k0 = 0; k0 < T.NS; k0 += 64) {
     const int k = k0 + lane;
@@ -410,6 +451,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
 #pragma unroll
   for (int v = 0; v < V; ++v) mean[v] = wsum(msum[v]) / T.wsum;
   bad = bad || check(mean);
+  }
   const bool troubled = wany(bad) || ratio > a.thr;
   if (lane == 0) {
     B.flags[e] = troubled ? 1 : 0;
