@@ -153,6 +153,16 @@ int fsg_create(const fsg_params* params, int32_t nblocks, const fsg_block_desc* 
 int fsg_residual(fsg_context* ctx, const void* const* U, void* const* out, int32_t mode,
                  int32_t terms, const fsg_boundary* boundary, void* stream);
 
+/* fsg_residual restricted to marked elements: mask[b] is an (E_b) int32
+ * array per block; the faces with a marked side and the marked elements are
+ * evaluated, and out[b] holds the result on the elements with mask != 0
+ * (bitwise equal to fsg_residual there; other elements' values are
+ * unspecified).  Serves the MOOD stage, whose viscous rhs is read only on
+ * the troubled elements (hybrid/subcell.py:876-886). */
+int fsg_residual_masked(fsg_context* ctx, const void* const* U, void* const* out, int32_t mode,
+                        int32_t terms, const int32_t* const* mask, const fsg_boundary* boundary,
+                        void* stream);
+
 /* One SSP-RK stage fused into the residual epilogue
  * (hybrid/time_integration.py:404-439 without a limiter):
  *   next = alpha * base + beta * (curr + dt * rhs(curr)),
@@ -305,6 +315,19 @@ int fsg_mood_stage(fsg_mood* m, const void* const* base, const void* const* curr
                    void* const* next, double dt, const void* const* dt_elem, double alpha,
                    double beta, const fsg_boundary* boundary, fsg_stage_diag* diag,
                    void* stream);
+
+/* fsg_mood_stage from one residual pass: total = rhs(curr) (inviscid and
+ * viscous terms together, fsg_residual mode RHS) replaces inv + visc; after
+ * detection the viscous rhs of curr is evaluated on the troubled elements
+ * only (fsg_residual_masked, into visc, which must hold (E_b, N, nvars)
+ * arrays; null when the viscosity is zero) and feeds their subcell
+ * recompute.  Same stage as hybrid/time_integration.py:404-439 up to the
+ * rounding of inv + visc. */
+int fsg_mood_stage_fused(fsg_mood* m, const void* const* base, const void* const* curr,
+                         const void* const* total, void* const* visc, void* const* work,
+                         void* const* next, double dt, const void* const* dt_elem, double alpha,
+                         double beta, const fsg_boundary* boundary, fsg_stage_diag* diag,
+                         void* stream);
 
 /* Number of troubled elements found by the last detection (device counter,
  * read synchronously on the stream). */

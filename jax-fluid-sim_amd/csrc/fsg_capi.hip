@@ -90,8 +90,8 @@ Phys<real> make_phys(const fsg_params& p, const fsg_boundary* b) {
 }
 
 template <int P, int V, typename real>
-int run_faces(fsg_context* c, const void* const* U, int terms, const Phys<real>& ph,
-              hipStream_t s) {
+int run_faces(fsg_context* c, const void* const* U, int terms, const int32_t* const* mask,
+              const Phys<real>& ph, hipStream_t s) {
   real* fo = static_cast<real*>(c->face_out);
   for (const fsg_group_desc& g : c->groups) {
     const fsg_block_desc& bo = c->blocks[g.owner_block];
@@ -113,6 +113,7 @@ int run_faces(fsg_context* c, const void* const* U, int terms, const Phys<real>&
     a.jinv_o = static_cast<const real*>(bo.jinv);
     a.scale_o = static_cast<const real*>(bo.scale);
     a.out_o = fo + g.out_owner;
+    if (mask) a.mask_o = mask[g.owner_block];
     int tn = -1;
     if (g.neighbor_block >= 0) {
       const fsg_block_desc& bn = c->blocks[g.neighbor_block];
@@ -125,6 +126,7 @@ int run_faces(fsg_context* c, const void* const* U, int terms, const Phys<real>&
       a.jinv_n = static_cast<const real*>(bn.jinv);
       a.scale_n = static_cast<const real*>(bn.scale);
       a.out_n = fo + g.out_neighbor;
+      if (mask) a.mask_n = mask[g.neighbor_block];
     }
     a.count = g.count;
     a.terms = terms;
@@ -140,7 +142,8 @@ int run_faces(fsg_context* c, const void* const* U, int terms, const Phys<real>&
 template <int P, int V, typename real>
 int run_elements(fsg_context* c, const void* const* U, const void* const* base, void* const* out,
                  int mode, int terms, double dt, const void* const* dt_elem, double alpha,
-                 double beta, fsg_stage_diag* diag, const Phys<real>& ph, hipStream_t s) {
+                 double beta, fsg_stage_diag* diag, const int32_t* const* mask,
+                 const Phys<real>& ph, hipStream_t s) {
   for (size_t b = 0; b < c->blocks.size(); ++b) {
     const fsg_block_desc& d = c->blocks[b];
     ElemArgs<real> a;
@@ -171,6 +174,7 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.mode = mode;
     a.terms = terms;
     a.diag = reinterpret_cast<StageDiag*>(diag);
+    a.mask = mask ? mask[b] : nullptr;
     long ev = timing_begin(c, int(c->groups.size() + b), s);
     const int mfma = c->element_mfma > 0 ? 1 : 0;
     int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s, mfma);
@@ -184,25 +188,27 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
 template <int P, int V, typename real>
 int run(fsg_context* c, const void* const* U, const void* const* base, void* const* out, int mode,
         int terms, double dt, const void* const* dt_elem, double alpha, double beta,
-        const fsg_boundary* bnd, fsg_stage_diag* diag, hipStream_t s) {
+        const fsg_boundary* bnd, fsg_stage_diag* diag, const int32_t* const* mask,
+        hipStream_t s) {
   Phys<real> ph = make_phys<real>(c->params, bnd);
-  int rc = run_faces<P, V, real>(c, U, terms, ph, s);
+  int rc = run_faces<P, V, real>(c, U, terms, mask, ph, s);
   if (rc) return rc;
   return run_elements<P, V, real>(c, U, base, out, mode, terms, dt, dt_elem, alpha, beta, diag,
-                                  ph, s);
+                                  mask, ph, s);
 }
 
 int dispatch(fsg_context* c, const void* const* U, const void* const* base, void* const* out,
              int mode, int terms, double dt, const void* const* dt_elem, double alpha,
-             double beta, const fsg_boundary* bnd, fsg_stage_diag* diag, hipStream_t s) {
+             double beta, const fsg_boundary* bnd, fsg_stage_diag* diag,
+             const int32_t* const* mask, hipStream_t s) {
   const fsg_params& p = c->params;
   const bool f64 = p.dtype == FSG_F64;
 #define FSG_RUN(P, V)                                                                          \
   if (p.degree == P && p.nvars == V)                                                          \
     return f64 ? run<P, V, double>(c, U, base, out, mode, terms, dt, dt_elem, alpha, beta, bnd, \
-                                   diag, s)                                                    \
+                                   diag, mask, s)                                              \
                : run<P, V, float>(c, U, base, out, mode, terms, dt, dt_elem, alpha, beta, bnd,  \
-                                  diag, s);
+                                  diag, mask, s);
   FSG_RUN(1, 5)
   FSG_RUN(2, 5)
   FSG_RUN(3, 5)
@@ -250,7 +256,18 @@ int fsg_residual(fsg_context* ctx, const void* const* U, void* const* out, int32
   if (!ctx || !U || !out) return fail(-10, "null argument");
   if (mode != FSG_MODE_RESIDUAL && mode != FSG_MODE_RHS) return fail(-17, "mode %d", mode);
   return dispatch(ctx, U, nullptr, out, mode, terms, 0.0, nullptr, 0.0, 0.0, boundary, nullptr,
-                  static_cast<hipStream_t>(stream));
+                  nullptr, static_cast<hipStream_t>(stream));
+}
+
+int fsg_residual_masked(fsg_context* ctx, const void* const* U, void* const* out, int32_t mode,
+                        int32_t terms, const int32_t* const* mask, const fsg_boundary* boundary,
+                        void* stream) {
+  if (!ctx || !U || !out || !mask) return fail(-10, "null argument");
+  if (mode != FSG_MODE_RESIDUAL && mode != FSG_MODE_RHS) return fail(-17, "mode %d", mode);
+  for (size_t b = 0; b < ctx->blocks.size(); ++b)
+    if (!mask[b] && ctx->blocks[b].count > 0) return fail(-10, "null mask of block %d", (int)b);
+  return dispatch(ctx, U, nullptr, out, mode, terms, 0.0, nullptr, 0.0, 0.0, boundary, nullptr,
+                  mask, static_cast<hipStream_t>(stream));
 }
 
 int fsg_rk_stage(fsg_context* ctx, const void* const* base, const void* const* curr,
@@ -259,7 +276,7 @@ int fsg_rk_stage(fsg_context* ctx, const void* const* base, const void* const* c
                  void* stream) {
   if (!ctx || !base || !curr || !next || !diag) return fail(-10, "null argument");
   return dispatch(ctx, curr, base, next, 2, terms, dt, dt_elem, alpha, beta, boundary, diag,
-                  static_cast<hipStream_t>(stream));
+                  nullptr, static_cast<hipStream_t>(stream));
 }
 
 int fsg_diag_reset(fsg_stage_diag* diag, void* stream) {

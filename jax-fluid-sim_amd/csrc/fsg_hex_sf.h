@@ -89,6 +89,7 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
 
   const int tid = threadIdx.x;
   const int64_t e = blockIdx.x;
+  if (a.mask && a.mask[e] == 0) return;   // whole workgroup: no barrier passed yet
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;

@@ -55,6 +55,7 @@ struct ElemArgs {
   int mode;               // 0 residual, 1 rhs, 2 stage
   int terms;
   StageDiag* diag;
+  const int32_t* mask;    // (E) evaluate only elements with mask != 0, or null: all
 };
 
 template <typename real>
@@ -83,6 +84,8 @@ struct FaceArgs {
   real* out_n;             // (F, FN, V+12)
   int64_t count;
   int terms;
+  const int32_t* mask_o;   // owner / neighbour element masks: evaluate only faces
+  const int32_t* mask_n;   // with a marked side, or null: all faces
 };
 
 template <typename real>
@@ -131,6 +134,7 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
 
   const int tid = threadIdx.x;
   const int64_t e = blockIdx.x;
+  if (a.mask && a.mask[e] == 0) return;   // whole workgroup: no barrier passed yet
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;
@@ -505,6 +509,9 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
 
   const int64_t eo = a.owner_elem[f];
   const int64_t en = NBR ? a.neighbor_elem[f] : 0;
+  // restricted evaluation: only faces with a marked side (one face per
+  // workgroup, so the whole workgroup leaves before any barrier)
+  if (a.mask_o && a.mask_o[eo] == 0 && !(NBR && a.mask_n[en] != 0)) return;
   const real n[3] = {a.geom[f * 4 + 0], a.geom[f * 4 + 1], a.geom[f * 4 + 2]};
   const real area = a.geom[f * 4 + 3];
   const int flags = NBR ? 0 : a.bflags[f];

@@ -1291,6 +1291,43 @@ int stage_impl(fsg_mood* m, const void* const* base, const void* const* curr,
   return launched("positivity");
 }
 
+// The stage from one residual pass (total = inv + visc evaluated together):
+// detection on the total, then the viscous rhs on the troubled elements only
+// (the subcell recompute reads it nowhere else).
+template <int V, typename real>
+int stage_fused_impl(fsg_mood* m, const void* const* base, const void* const* curr,
+                     const void* const* total, void* const* visc, void* const* work,
+                     void* const* next, double dt, const void* const* dt_elem, double alpha,
+                     double beta, const fsg_boundary* bnd, fsg_stage_diag* diag, hipStream_t s) {
+  Args<real> a = base_args<real>(m);
+  const int nb = a.nblocks;
+  Phys<real> ph = phys_of<real>(m, bnd);
+  set_ptrs(a.base, base, nb);
+  set_ptrs(a.curr, curr, nb);
+  set_ptrs(a.inv, total, nb);
+  set_ptrs(a.dte, dt_elem, nb);
+  set_ptrs(a.work, (const void* const*)work, nb);
+  set_ptrs(a.out, (const void* const*)next, nb);
+  a.dt = real(dt);
+  a.alpha = real(alpha);
+  a.beta = real(beta);
+  a.stage = 1;
+  a.diag = reinterpret_cast<StageDiag*>(diag);
+  int rc;
+  if ((rc = run_detect<V, real>(m, a, ph, s))) return rc;
+  if (visc) {
+    const int32_t* mask[MAXB];
+    for (int b = 0; b < nb; ++b) mask[b] = m->flags[b];
+    if ((rc = fsg_residual_masked(m->ctx, curr, visc, FSG_MODE_RHS, FSG_TERM_VISCOUS, mask, bnd,
+                                  s)))
+      return rc;
+    set_ptrs(a.visc, (const void* const*)visc, nb);
+  }
+  if ((rc = run_recompute_core<V, real>(m, a, ph, s))) return rc;
+  launch_positivity<V, real>(m, a, ph, 1, s);
+  return launched("positivity");
+}
+
 #define FSG_MOOD_DISPATCH(call, args)                                        \
   do {                                                                       \
     const int nv = m->ctx->params.nvars;                                     \
@@ -1371,6 +1408,17 @@ int fsg_mood_stage(fsg_mood* m, const void* const* base, const void* const* curr
   hipStream_t s = static_cast<hipStream_t>(stream);
   FSG_MOOD_DISPATCH(stage_impl, (m, base, curr, inv, visc, work, next, dt, dt_elem, alpha,
                                  beta, boundary, diag, s));
+}
+
+int fsg_mood_stage_fused(fsg_mood* m, const void* const* base, const void* const* curr,
+                         const void* const* total, void* const* visc, void* const* work,
+                         void* const* next, double dt, const void* const* dt_elem, double alpha,
+                         double beta, const fsg_boundary* boundary, fsg_stage_diag* diag,
+                         void* stream) {
+  if (!m || !base || !curr || !total || !work || !next) return fsg_fail(-10, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FSG_MOOD_DISPATCH(stage_fused_impl, (m, base, curr, total, visc, work, next, dt, dt_elem,
+                                       alpha, beta, boundary, diag, s));
 }
 
 int fsg_mood_count(fsg_mood* m, int64_t* count, void* stream) {

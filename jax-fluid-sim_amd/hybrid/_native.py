@@ -98,6 +98,9 @@ def lib():
         L.fsg_residual.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, i32,
                                    ctypes.POINTER(Boundary), vp]
         L.fsg_residual.restype = ctypes.c_int
+        L.fsg_residual_masked.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, i32,
+                                          ctypes.POINTER(vp), ctypes.POINTER(Boundary), vp]
+        L.fsg_residual_masked.restype = ctypes.c_int
         L.fsg_rk_stage.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
                                    f64, ctypes.POINTER(vp), f64, f64, i32,
                                    ctypes.POINTER(Boundary), vp, vp]
@@ -140,9 +143,12 @@ def lib():
         L.fsg_mood_positivity.argtypes = [vp, pp, pp, vp]
         L.fsg_mood_recompute.argtypes = [vp, pp, pp, pp, pp, f64, pp, i32, vp, pp, vp]
         L.fsg_mood_stage.argtypes = [vp, pp, pp, pp, pp, pp, pp, f64, pp, f64, f64, vp, vp, vp]
+        L.fsg_mood_stage_fused.argtypes = [vp, pp, pp, pp, pp, pp, pp, f64, pp, f64, f64, vp, vp,
+                                           vp]
         L.fsg_mood_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), vp]
         for name in ("fsg_mood_create", "fsg_mood_troubled", "fsg_mood_positivity",
-                     "fsg_mood_recompute", "fsg_mood_stage", "fsg_mood_count"):
+                     "fsg_mood_recompute", "fsg_mood_stage", "fsg_mood_stage_fused",
+                     "fsg_mood_count"):
             getattr(L, name).restype = ctypes.c_int
         f32 = ctypes.c_float
         L.fsg_mlp_linear_forward.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]

@@ -354,5 +354,26 @@ class HybridMOODSubcellLimiter:
             float(dt), dt_elem, float(alpha), float(beta), ctypes.byref(bnd), diag,
             nat.stream_handle(self.solver.device)))
 
+    def stage_fused(self, base, curr, total, visc, work, nxt, dt, alpha, beta, boundary_state,
+                    diag):
+        """The MOOD branch from one residual pass: ``total`` = rhs(curr);
+        the viscous rhs is evaluated into ``visc`` (or skipped when None) on
+        the troubled elements only, after detection."""
+        op = self.operator
+        types = op.block_types
+        dt_elem = None
+        if isinstance(dt, dict):
+            dts = [dt[t].reshape(-1).to(self.solver.dtype).contiguous() for t in types]
+            dt_elem = nat.ptr_array(dts)
+            dt = 0.0
+        bnd = op._boundary(boundary_state)
+        nat.check(nat.lib().fsg_mood_stage_fused(
+            self._ctx().ctx, nat.ptr_array(op._inputs(base)), nat.ptr_array(op._inputs(curr)),
+            nat.ptr_array([total[t] for t in types]),
+            nat.ptr_array([visc[t] for t in types]) if visc is not None else None,
+            nat.ptr_array([work[t] for t in types]), nat.ptr_array([nxt[t] for t in types]),
+            float(dt), dt_elem, float(alpha), float(beta), ctypes.byref(bnd), diag,
+            nat.stream_handle(self.solver.device)))
+
     def __call__(self, states):
         return self.positivity(states)

@@ -104,3 +104,22 @@ class HybridNavierStokesOperator(HybridEulerOperator):
 
     def rhs(self, states, boundary_state=None):
         return self._call(states, nat.MODE_RHS, self.rhs_terms(), boundary_state)
+
+    def viscous_rhs_masked(self, states, masks, boundary_state=None):
+        """viscous_rhs evaluated on the elements whose int32 mask entry is
+        non-zero (bitwise equal to viscous_rhs there; other entries are
+        unspecified)."""
+        U = self._inputs(states)
+        out = [torch.empty_like(u) for u in U]
+        ms = []
+        for u, t in zip(U, self.block_types):
+            m = masks[t]
+            if m.dtype != torch.int32 or m.numel() != u.shape[0] or not m.is_contiguous():
+                raise ValueError(f"{t} mask must be a contiguous int32 tensor of {u.shape[0]}")
+            ms.append(m)
+        b = self._boundary(boundary_state)
+        nat.check(nat.lib().fsg_residual_masked(
+            self.device_mesh.ctx, nat.ptr_array(U), nat.ptr_array(out), nat.MODE_RHS,
+            nat.TERM_VISCOUS, nat.ptr_array(ms), ctypes.byref(b),
+            nat.stream_handle(self.solver.device)))
+        return dict(zip(self.block_types, out))

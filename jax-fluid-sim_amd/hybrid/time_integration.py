@@ -47,6 +47,11 @@ class HybridSSPRK3:
         self.maximum_rejections = int(maximum_rejections)
         self._diag = StageDiagnostics(self.solver.device)
         self._buffers = None
+        # MOOD stages from one residual pass (inviscid and viscous together,
+        # the viscous rhs re-evaluated on troubled elements only); False
+        # evaluates inviscid_rhs and viscous_rhs separately as the reference
+        # does (the two differ by the rounding of inv + visc)
+        self.fused_mood = True
         self.characteristic_lengths = self._characteristic_lengths()
 
     def _characteristic_lengths(self):
@@ -117,6 +122,14 @@ class HybridSSPRK3:
         op = self.operator
         stream = nat.stream_handle(self.solver.device)
         self._diag.reset(stream)
+        viscous = hasattr(op, "inviscid_rhs") and self.solver.dynamic_viscosity > 0.0
+        if viscous and self.fused_mood:
+            total = op.rhs(curr, boundary_state)
+            visc = {t: torch.empty_like(curr[t]) for t in op.block_types}
+            work = {t: torch.empty_like(curr[t]) for t in op.block_types}
+            self.limiter.stage_fused(base, curr, total, visc, work, nxt, dt, alpha, beta,
+                                     boundary_state, nat.ptr(self._diag.buf))
+            return
         if hasattr(op, "inviscid_rhs"):
             inv = op.inviscid_rhs(curr, boundary_state)
             # a zero viscosity gives an exactly zero viscous rhs: skip it

@@ -176,8 +176,13 @@ def test_owner_indexing_conserves(recon):
     assert np.abs(_total(oop, got) - _total(oop, cand)).max() <= 1e-12 * tot
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("name,p", [("split", 2), ("periodic", 2), ("split", 3)])
-def test_mood_stage_matches(name, p):
+def test_mood_stage_matches(name, p, fused):
+    """Both stage forms against the oracle: one residual pass with the
+    viscous rhs re-evaluated on the troubled elements (fused), and the
+    reference's separate inviscid and viscous passes."""
+    print(f"case {name} p={p} fused={fused}", flush=True)
     solver, op, b, lim, oop, ob, olim, states = _case(name, p)
     curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x, seed=9))
             for t, x in oop.geometry.solution_node_coordinates.items()}
@@ -189,7 +194,9 @@ def test_mood_stage_matches(name, p):
 
     want, wvalid, wr, wp = rk.rk_stage(states, curr, dt, 0.75, 0.25, ob)
     drk = HybridSSPRK3(op, limiter=lim)
+    drk.fused_mood = fused
     got, valid, rmin, pmin = drk.rk_stage(_dev(solver, states), _dev(solver, curr), dt, 0.75, 0.25, b)
+    assert lim.troubled_count() > 0
     assert bool(valid) == bool(wvalid)
     assert abs(float(rmin) - wr) <= 1e-12 * abs(wr) and abs(float(pmin) - wp) <= 1e-10 * abs(wp)
     assert np.all(rel_l2(_np(got), want) <= _bar(oracle, curr))
@@ -228,3 +235,55 @@ def test_fp32_path(name, p):
                                                          dt, 0.75, 0.25, b)
     assert bool(gv) == bool(wv)
     assert np.all(rel_l2(_np(g), w) <= FP32_TOL)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("name,p", [("split", 2), ("split", 3)])
+def test_masked_viscous_rhs_is_bitwise(name, p, dtype):
+    """The viscous rhs restricted to marked elements equals the full viscous
+    rhs bit for bit on those elements, on every element path (the masked
+    pass re-evaluates every face of a marked element)."""
+    print(f"case {name} p={p} {dtype} masked viscous rhs", flush=True)
+    solver, op, b, lim, oop, ob, olim, states = _case(name, p, dtype=dtype)
+    U = _dev(solver, states)
+    g = torch.Generator().manual_seed(3)
+    masks = {t: (torch.rand(U[t].shape[0], generator=g) < 0.3).to(torch.int32).cuda()
+             for t in op.block_types}
+    dm = op.device_mesh
+    try:
+        for mfma, sf in ((None, None), (True, False), (False, False), (False, True)):
+            dm.set_element_path(mfma=mfma)
+            dm.set_hex_sum_factorisation(sf)
+            full = op.viscous_rhs(U, b)
+            op.rhs(U, b)     # leaves combined face records behind
+            part = op.viscous_rhs_masked(U, masks, b)
+            for t in op.block_types:
+                m = masks[t].bool()
+                assert int(m.sum()) > 0
+                assert torch.equal(part[t][m], full[t][m]), (t, mfma, sf)
+    finally:
+        dm.set_element_path(None)
+        dm.set_hex_sum_factorisation(None)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_fused_stage_matches_split_stage(dtype):
+    """One residual pass per MOOD stage against the reference's two passes:
+    the same troubled elements, and states equal to the rounding of
+    inv + visc (1e-13 relative in FP64, 1e-5 in FP32)."""
+    solver, op, b, lim, oop, ob, olim, states = _case("split", 2, dtype=dtype)
+    curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x, seed=9))
+            for t, x in oop.geometry.solution_node_coordinates.items()}
+    S, C = _dev(solver, states), _dev(solver, curr)
+    drk = HybridSSPRK3(op, limiter=lim)
+    out, flags = {}, {}
+    for fused in (True, False):
+        drk.fused_mood = fused
+        out[fused] = _np(drk.rk_stage(S, C, 2e-3, 0.75, 0.25, b)[0])
+        flags[fused] = {t: v.clone() for t, v in
+                        ((t, lim._ctx().tables.flags[k]) for k, t in enumerate(op.block_types))}
+    for t in op.block_types:
+        assert torch.equal(flags[True][t], flags[False][t]), t
+    assert sum(int(f.sum()) for f in flags[True].values()) > 0
+    tol = 1e-13 if dtype == torch.float64 else 1e-5
+    assert np.all(rel_l2(out[True], out[False]) <= tol), rel_l2(out[True], out[False])
