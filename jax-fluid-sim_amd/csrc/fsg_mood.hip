@@ -324,25 +324,35 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
   const BlockT<real>& B = a.blocks[b];
   const TypeT<real>& T = a.types[B.type];
   const int N = T.N;
-  if (lane < N) {
-    const int64_t o = (e * N + lane) * V;
-    real u[V];
+  {
+    // element values in node-major order, one coalesced pass over N V
+    real* su = &sU[w][0][0];
+    const int64_t o = e * N * V;
     if (a.stage) {
       const real h = a.dte[b] ? a.dte[b][e] : a.dt;
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        real r = a.inv[b][o + v];
-        if (a.visc[b]) r = r + a.visc[b][o + v];
-        const real eu = a.curr[b][o + v] + h * r;
-        a.work[b][o + v] = eu;
-        u[v] = a.alpha * a.base[b][o + v] + a.beta * eu;
+      const real* inv = a.inv[b] + o;
+      const real* vis = a.visc[b] ? a.visc[b] + o : nullptr;
+      const real* cur = a.curr[b] + o;
+      const real* bas = a.base[b] + o;
+      real* wrk = a.work[b] + o;
+      for (int k = lane; k < N * V; k += 64) {
+        real r = inv[k];
+        if (vis) r = r + vis[k];
+        const real eu = cur[k] + h * r;
+        wrk[k] = eu;
+        su[k] = a.alpha * bas[k] + a.beta * eu;
       }
     } else {
-#pragma unroll
-      for (int v = 0; v < V; ++v) u[v] = a.det_src[b][o + v];
+      const real* src = a.det_src[b] + o;
+      for (int k = lane; k < N * V; k += 64) su[k] = src[k];
     }
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (lane < N) {
+    real u[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) sU[w][lane][v] = u[v];
+    for (int v = 0; v < V; ++v) u[v] = sU[w][lane][v];
     fsg::Prim<V, real> pr = fsg::c2p<V, real>(u, ph.gamma);
     sm[w][lane][0] = pr.rho;
     sm[w][lane][1] = pr.p;
@@ -711,52 +721,75 @@ __global__ __launch_bounds__(64) void subcell_kernel(Args<real> a, Phys<real> ph
   }
 }
 
-// Mean correction of non-troubled neighbours (hybrid/subcell.py:891-944):
-// one wave per element.
+// Mean correction of non-troubled neighbours (hybrid/subcell.py:891-944).
+// Each lane first tests one element (not troubled, with a troubled face
+// neighbour); the wave then corrects the elements it found one at a time,
+// with the face points spread over the lanes.  Nothing to do without
+// troubled elements.
 template <int V, typename real>
 __global__ __launch_bounds__(256) void correct_kernel(Args<real> a, Phys<real> ph, int64_t total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t g = (int64_t)blockIdx.x * 4 + w;
-  if (g >= total) return;
-  int64_t e;
-  const int b = locate(a, g, &e);
-  const BlockT<real>& B = a.blocks[b];
-  if (B.flags[e]) return;
-  const TypeT<real>& T = a.types[B.type];
-  real corr[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) corr[v] = real(0);
-  bool any = false;
-  const real h = a.dte[b] ? a.dte[b][e] : a.dt;
-  const real evol = B.detw[e] * T.volsum;
-  for (int fe = 0; fe < B.max_faces; ++fe) {
-    const int32_t code = B.elem_faces[(e * B.max_faces + fe) * 2];
-    if (code < 0) continue;
-    const GroupT<real>& G = a.groups[code & 0xffff];
-    if (G.neighbor_block < 0) continue;       // the boundary side is never troubled
-    const int side = (code >> 24) & 1;
-    const int64_t i = B.elem_faces[(e * B.max_faces + fe) * 2 + 1];
-    const BlockT<real>& Bo = a.blocks[side == 0 ? G.neighbor_block : G.owner_block];
-    const int64_t other = side == 0 ? G.neighbor_elem[i] : G.owner_elem[i];
-    if (!Bo.flags[other]) continue;
-    any = true;
-    real nf[V], of[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      nf[v] = real(0);
-      of[v] = real(0);
-    }
-    if (lane < G.K) pair_flux<V, real>(a, ph, G, i, lane, nf);
-    if (lane < G.Qf) dg_point_flux<V, real>(a, ph, G, i, lane, of);
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const real d = wsum(nf[v]) - wsum(of[v]);
-      corr[v] += (side == 0 ? -h : h) * d / evol;
+  if (*a.counter == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t g0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  bool need = false;
+  if (g0 + lane < total) {
+    int64_t e;
+    const int b = locate(a, g0 + lane, &e);
+    const BlockT<real>& B = a.blocks[b];
+    if (!B.flags[e]) {
+      for (int fe = 0; fe < B.max_faces && !need; ++fe) {
+        const int32_t code = B.elem_faces[(e * B.max_faces + fe) * 2];
+        if (code < 0) continue;
+        const GroupT<real>& G = a.groups[code & 0xffff];
+        if (G.neighbor_block < 0) continue;       // the boundary side is never troubled
+        const int side = (code >> 24) & 1;
+        const int64_t i = B.elem_faces[(e * B.max_faces + fe) * 2 + 1];
+        const BlockT<real>& Bo = a.blocks[side == 0 ? G.neighbor_block : G.owner_block];
+        const int64_t other = side == 0 ? G.neighbor_elem[i] : G.owner_elem[i];
+        need = Bo.flags[other] != 0;
+      }
     }
   }
-  if (!any) return;
-  real* out = a.work[b] + e * T.N * V;
-  for (int k = lane; k < T.N * V; k += 64) out[k] = out[k] + corr[k % V];
+  unsigned long long todo = __ballot(need);
+  while (todo) {
+    const int l = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    int64_t e;
+    const int b = locate(a, g0 + l, &e);
+    const BlockT<real>& B = a.blocks[b];
+    const TypeT<real>& T = a.types[B.type];
+    real corr[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) corr[v] = real(0);
+    const real h = a.dte[b] ? a.dte[b][e] : a.dt;
+    const real evol = B.detw[e] * T.volsum;
+    for (int fe = 0; fe < B.max_faces; ++fe) {
+      const int32_t code = B.elem_faces[(e * B.max_faces + fe) * 2];
+      if (code < 0) continue;
+      const GroupT<real>& G = a.groups[code & 0xffff];
+      if (G.neighbor_block < 0) continue;
+      const int side = (code >> 24) & 1;
+      const int64_t i = B.elem_faces[(e * B.max_faces + fe) * 2 + 1];
+      const BlockT<real>& Bo = a.blocks[side == 0 ? G.neighbor_block : G.owner_block];
+      const int64_t other = side == 0 ? G.neighbor_elem[i] : G.owner_elem[i];
+      if (!Bo.flags[other]) continue;
+      real nf[V], of[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        nf[v] = real(0);
+        of[v] = real(0);
+      }
+      if (lane < G.K) pair_flux<V, real>(a, ph, G, i, lane, nf);
+      if (lane < G.Qf) dg_point_flux<V, real>(a, ph, G, i, lane, of);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const real d = wsum(nf[v]) - wsum(of[v]);
+        corr[v] += (side == 0 ? -h : h) * d / evol;
+      }
+    }
+    real* out = a.work[b] + e * T.N * V;
+    for (int k = lane; k < T.N * V; k += 64) out[k] = out[k] + corr[k % V];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1199,11 +1232,11 @@ void launch_positivity(fsg_mood* m, const Args<real>& a, const Phys<real>& ph, i
 
 template <int V, typename real>
 int run_recompute_core(fsg_mood* m, Args<real>& a, const Phys<real>& ph, hipStream_t s) {
-  const unsigned g4 = (unsigned)((m->total + 3) / 4);
+  const unsigned g256 = (unsigned)((m->total + 255) / 256);
   const unsigned gs = (unsigned)(m->total < SUBCELL_GRID ? m->total : SUBCELL_GRID);
   hipLaunchKernelGGL((subcell_kernel<V, real>), dim3(gs), dim3(64), 0, s, a, ph);
   if (int rc = launched("MOOD subcell")) return rc;
-  hipLaunchKernelGGL((correct_kernel<V, real>), dim3(g4), dim3(256), 0, s, a, ph, m->total);
+  hipLaunchKernelGGL((correct_kernel<V, real>), dim3(g256), dim3(256), 0, s, a, ph, m->total);
   return launched("MOOD correction");
 }
 
