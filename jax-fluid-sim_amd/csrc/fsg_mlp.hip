@@ -199,6 +199,12 @@ struct InferConsts {
   double o_atoms[NOUT];
 };
 
+// Constraint rows of the projection per unit mass (N atoms, O atoms,
+// charge), built once on the host: A[i][s] = (atoms or charge)_s / M_s.
+struct ProjRows {
+  double A[3][NOUT];
+};
+
 __device__ __forceinline__ bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -219,9 +225,12 @@ __device__ __forceinline__ bf16x8 to_b(f32x4 lo, f32x4 hi, const float* blo, con
   return r;
 }
 
-// One wavefront = 16 cells.  Block = 4 wavefronts; the weights of all layers
-// are staged once per block in LDS and reused by a grid-stride loop.
-__global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ Wp,
+// One wavefront = 16 cells.  Block = IW wavefronts sharing the weights of
+// all layers, staged once per block in LDS (78 KB: one block per CU, so the
+// block is wide to keep four waves per SIMD); each wave runs its own
+// grid-stride loop over 16-cell tiles.
+constexpr int IW = 16;
+__global__ __launch_bounds__(IW * 64) void infer_kernel(const __bf16* __restrict__ Wp,
                                                     const float* __restrict__ bias,
                                                     const InferConsts cstv,
                                                     const double* __restrict__ rho_in,
@@ -231,10 +240,10 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
                                                     int32_t* __restrict__ flag, int64_t n) {
   __shared__ __bf16 sW[W_TOTAL];
   __shared__ float sb[3 * HID + 16];
-  __shared__ float sO[4][16][17];
-  for (int k = threadIdx.x; k < W_TOTAL / 8; k += 256)
+  __shared__ float sO[IW][16][17];
+  for (int k = threadIdx.x; k < W_TOTAL / 8; k += IW * 64)
     reinterpret_cast<bf16x8*>(sW)[k] = reinterpret_cast<const bf16x8*>(Wp)[k];
-  for (int k = threadIdx.x; k < 3 * HID + 16; k += 256) sb[k] = bias[k];
+  for (int k = threadIdx.x; k < 3 * HID + 16; k += IW * 64) sb[k] = bias[k];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cellw = lane & 15;
@@ -244,9 +253,9 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
   const __bf16* W4 = W3 + W3_BLK * BLK;
   const InferConsts* cst = &cstv;
   const int64_t ntiles = (n + 15) / 16;
-  // Block-uniform trip count (the loop body holds block barriers).
-  for (int64_t base = (int64_t)blockIdx.x * 4; base < ntiles; base += (int64_t)gridDim.x * 4) {
-    const int64_t tile = base + wave;
+  // no block barriers below: each wave's loop is its own
+  for (int64_t tile = (int64_t)blockIdx.x * IW + wave; tile < ntiles;
+       tile += (int64_t)gridDim.x * IW) {
     const int64_t c = tile * 16 + cellw;
     const bool valid = c < n;
     // Features of this lane's cell, in the K-slot order of the B operand.
@@ -313,7 +322,8 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
       int row = (lane / 16) * 4 + j;
       sO[wave][cellw][row] = out[j] + sb[3 * HID + row];
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     // The network output of each cell goes to its own rho_out slot as NOUT
     // floats (the slot holds NOUT doubles); project_kernel reads them there.
     if (lane < 16 && valid) {
@@ -321,7 +331,8 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
 #pragma unroll
       for (int s = 0; s < NOUT; ++s) o[s] = sO[wave][lane][s];
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const __bf16* __restrict__ W
 // charge and mass constraints of the input (FP64), one cell per thread; the
 // network output is read from the cell's rho_out slot (NOUT floats written by
 // infer_kernel) before the slot is overwritten with the result.
-__global__ __launch_bounds__(64) void project_kernel(const InferConsts cstv,
+__global__ __launch_bounds__(64) void project_kernel(const InferConsts cstv, const ProjRows rows,
                                                      const double* __restrict__ rho_in,
                                                      const double* __restrict__ T_in,
                                                      const double* __restrict__ dt_cell,
@@ -376,13 +387,8 @@ __global__ __launch_bounds__(64) void project_kernel(const InferConsts cstv,
       // the integrator's own final rescale (hybrid/chemistry/integrator.py:102),
       // which keeps the charge exact and moves N and O by the change of Y_e,
       // as the integrator itself does.
-      double A[3][NOUT], cvec[3];
-#pragma unroll
-      for (int s = 0; s < NOUT; ++s) {
-        A[0][s] = cst->n_atoms[s] / cst->molar_mass[s];
-        A[1][s] = cst->o_atoms[s] / cst->molar_mass[s];
-        A[2][s] = cst->charge[s] / cst->molar_mass[s];
-      }
+      const double (*A)[NOUT] = rows.A;   // uniform: held in scalar registers
+      double cvec[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         cvec[i] = 0.0;
@@ -593,15 +599,21 @@ int fsg_mlp_infer(const void* packed_weights, const float* biases, const void* c
   fsgmlp::InferConsts c;
   memcpy(&c, consts, sizeof c);
   int64_t tiles = (n + 15) / 16;
-  int64_t want = (tiles + 3) / 4;
+  int64_t want = (tiles + fsgmlp::IW - 1) / fsgmlp::IW;
   unsigned grid = (unsigned)(blocks > 0 && want > blocks ? blocks : want);
-  hipLaunchKernelGGL(fsgmlp::infer_kernel, dim3(grid), dim3(256), 0,
+  hipLaunchKernelGGL(fsgmlp::infer_kernel, dim3(grid), dim3(fsgmlp::IW * 64), 0,
                      static_cast<hipStream_t>(stream),
                      static_cast<const __bf16*>(packed_weights), biases, c, rho_in, T_in,
                      dt_cell, dt, rho_out, fallback, n);
   if (int rc = launched()) return rc;
+  fsgmlp::ProjRows rows;
+  for (int s = 0; s < fsgmlp::NOUT; ++s) {
+    rows.A[0][s] = c.n_atoms[s] / c.molar_mass[s];
+    rows.A[1][s] = c.o_atoms[s] / c.molar_mass[s];
+    rows.A[2][s] = c.charge[s] / c.molar_mass[s];
+  }
   hipLaunchKernelGGL(fsgmlp::project_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
-                     static_cast<hipStream_t>(stream), c, rho_in, T_in, dt_cell, dt, rho_out,
+                     static_cast<hipStream_t>(stream), c, rows, rho_in, T_in, dt_cell, dt, rho_out,
                      fallback, n);
   return launched();
 }
