@@ -211,6 +211,14 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// SiLU of the inference path: the hardware reciprocal (1 ulp) in place of
+// the IEEE division; the result is rounded to BF16 next, so the two agree
+// after rounding but for rare ties, at a fraction of the VALU work (the
+// activations are the kernel's largest VALU cost).
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
 // Converts two accumulator tiles (rows 0-15 and 16-31 of a K=32 step) to the
 // B operand of the next layer after bias + SiLU.
 __device__ __forceinline__ bf16x8 to_b(f32x4 lo, f32x4 hi, const float* blo, const float* bhi,
@@ -219,8 +227,8 @@ __device__ __forceinline__ bf16x8 to_b(f32x4 lo, f32x4 hi, const float* blo, con
   const int base = (l / 16) * 4;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    r[j] = (__bf16)silu(lo[j] + blo[base + j]);
-    r[4 + j] = (__bf16)silu(hi[j] + bhi[base + j]);
+    r[j] = (__bf16)silu_fast(lo[j] + blo[base + j]);
+    r[4 + j] = (__bf16)silu_fast(hi[j] + bhi[base + j]);
   }
   return r;
 }
@@ -252,15 +260,17 @@ __global__ __launch_bounds__(IW * 64) void infer_kernel(const __bf16* __restrict
   const __bf16* W3 = W2 + W2_BLK * BLK;
   const __bf16* W4 = W3 + W3_BLK * BLK;
   const InferConsts* cst = &cstv;
-  const int64_t ntiles = (n + 15) / 16;
-  // no block barriers below: each wave's loop is its own
-  for (int64_t tile = (int64_t)blockIdx.x * IW + wave; tile < ntiles;
-       tile += (int64_t)gridDim.x * IW) {
-    const int64_t c = tile * 16 + cellw;
-    const bool valid = c < n;
-    // Features of this lane's cell, in the K-slot order of the B operand.
-    float feat[K1];
+  const int64_t ngroups = (n + 63) / 64;
+  // no block barriers below: each wave's loop is its own.  A wave takes 64
+  // cells: lane l computes the features of cell l once, then the four
+  // 16-cell tiles run through the network in turn, each lane fetching its
+  // B-operand slots from the lane that owns the tile's cell.
+  for (int64_t grp = (int64_t)blockIdx.x * IW + wave; grp < ngroups;
+       grp += (int64_t)gridDim.x * IW) {
+    float feat[NIN];
     {
+      const int64_t c = grp * 64 + lane;
+      const bool valid = c < n;
       double r[NOUT], rt = 0.0;
 #pragma unroll
       for (int s = 0; s < NOUT; ++s) {
@@ -276,19 +286,30 @@ __global__ __launch_bounds__(IW * 64) void infer_kernel(const __bf16* __restrict
       for (int s = 0; s < NOUT; ++s) f[2 + s] = (float)(r[s] / rt);
       f[13] = (float)log(h);
 #pragma unroll
-      for (int k = 0; k < K1; ++k) feat[k] = 0.f;
-#pragma unroll
       for (int k = 0; k < NIN; ++k) feat[k] = (f[k] - cst->mean[k]) * cst->inv_std[k];
     }
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+    const int64_t c = grp * 64 + t * 16 + cellw;
+    const bool valid = c < n;
+    // B operand: slot j < 4 of lane l holds feature perm_k(l, j) = 4 (l / 16)
+    // + j of cell 16 t + l % 16 (slots 4-7 index the zero padding 16-31)
     bf16x8 x;
+    {
+      const int gq = lane >> 4, src = t * 16 + cellw;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      // feat index for (lane, slot j) is perm_k(lane, j); select by unrolled compare.
-      int kk = perm_k(lane, j);
-      float v = 0.f;
+      for (int j = 0; j < 4; ++j) {
+        float v = 0.f;
 #pragma unroll
-      for (int k = 0; k < NIN; ++k) v = (kk == k) ? feat[k] : v;
-      x[j] = (__bf16)v;
+        for (int g = 0; g < 4; ++g) {
+          if (4 * g + j < NIN) {
+            const float w = __shfl(feat[4 * g + j], src, 64);
+            v = gq == g ? w : v;
+          }
+        }
+        x[j] = (__bf16)v;
+        x[4 + j] = (__bf16)0.f;
+      }
     }
     // Layer 1: 8 output tiles, one K step.
     f32x4 h1[8];
@@ -333,6 +354,7 @@ __global__ __launch_bounds__(IW * 64) void infer_kernel(const __bf16* __restrict
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
   }
 }
 
@@ -448,6 +470,13 @@ __global__ __launch_bounds__(64) void project_kernel(const InferConsts cstv, con
 #pragma unroll
         for (int k = 0; k < 3; ++k) lam[k] -= damp * d[k];
         if (!(step > 1e-14)) break;   // converged (or NaN: caught by the checks below)
+      }
+      // the input is read again rather than held in registers through the
+      // iterations (the same values: r and Y0 = r / rt bit for bit)
+#pragma unroll
+      for (int s = 0; s < NOUT; ++s) {
+        r[s] = __builtin_nontemporal_load(rho_in + c * NOUT + s);
+        Y0[s] = r[s] / rt;
       }
       double Yf[NOUT], zf = 0.0;
 #pragma unroll
@@ -598,8 +627,8 @@ int fsg_mlp_infer(const void* packed_weights, const float* biases, const void* c
   if (n <= 0) return 0;
   fsgmlp::InferConsts c;
   memcpy(&c, consts, sizeof c);
-  int64_t tiles = (n + 15) / 16;
-  int64_t want = (tiles + fsgmlp::IW - 1) / fsgmlp::IW;
+  int64_t groups = (n + 63) / 64;
+  int64_t want = (groups + fsgmlp::IW - 1) / fsgmlp::IW;
   unsigned grid = (unsigned)(blocks > 0 && want > blocks ? blocks : want);
   hipLaunchKernelGGL(fsgmlp::infer_kernel, dim3(grid), dim3(fsgmlp::IW * 64), 0,
                      static_cast<hipStream_t>(stream),
