@@ -491,15 +491,27 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   constexpr int UG = (WIDE || (HQ && TO == HEX)) ? 4 : NO;
   constexpr int UGN = (WIDE || (HQ && TN == HEX)) ? 4 : NN;
 
-  __shared__ real sUo[NO * V], sUn[NN * V], svo[NO * 4], svn[NN * 4];
-  __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
-  __shared__ real sAo[QF * V], sAn[NBR ? QF * V : 1];
+  // LDS: the staged states (anchor node 0, then the face-support nodes) and
+  // the anchored traces are dead after the Euler flux (phase 1b); the face
+  // gradients (phases 3-4) and the sum-factorisation scratch reuse them.
+  constexpr int SO = FO + 1, SN = NBR ? FN + 1 : 1;
   // BR2 face gradients by sum factorisation on hexahedral quad sides
   constexpr bool SFO = FT == QUAD && SUB == 0 && TO == HEX && P >= 2;
   constexpr bool SFN = FT == QUAD && SUB == 0 && TN == HEX && P >= 2;
   constexpr int SFN1 = P + 1, SFQ1 = P == 2 ? 3 : 4;
-  __shared__ real sfs[(SFO || SFN) ? 2 * SFN1 * SFN1 * 4 + 3 * SFN1 * SFQ1 * 4 : 1];
-  __shared__ real sbo[FO * 12], sbn[FN * 12], sgo[QF * 12], sgn[QF * 12];
+  constexpr int SFS = (SFO || SFN) ? 2 * SFN1 * SFN1 * 4 + 3 * SFN1 * SFQ1 * 4 : 1;
+  constexpr int NU = (SO + SN) * V > 2 * QF * 12 ? (SO + SN) * V : 2 * QF * 12;
+  constexpr int NA = (NBR ? 2 : 1) * QF * V > SFS ? (NBR ? 2 : 1) * QF * V : SFS;
+  __shared__ real sUU[NU], sAA[NA], svo[NO * 4], svn[NN * 4];
+  __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
+  __shared__ real sbo[FO * 12], sbn[FN * 12];
+  real* const sUo = sUU;                 // [SO][V], slot 0 = node 0, slot 1 + k = support node k
+  real* const sUn = sUU + SO * V;        // [SN][V]
+  real* const sgo = sUU;                 // [QF][12] after phase 1b
+  real* const sgn = sUU + QF * 12;
+  real* const sAo = sAA;                 // [QF][V]
+  real* const sAn = sAA + QF * V;
+  real* const sfs = sAA;                 // after phase 1b
 
   const int tid = threadIdx.x;
   const int64_t f = blockIdx.x;
@@ -526,14 +538,36 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     supn = a.neighbor_sup + tn_id * FN;
   }
 
-  for (int k = tid; k < NO * V; k += 64) sUo[k] = a.Uo[eo * (NO * V) + k];
-  if constexpr (NBR)
-    for (int k = tid; k < NN * V; k += 64) sUn[k] = a.Un[en * (NN * V) + k];
-  __syncthreads();
+  // stage node 0 (the anchor) and the face-support nodes of each side; the
+  // viscous variables of every node come from the flow columns directly
+  {
+    const real* Ue = a.Uo + eo * (NO * V);
+    for (int k = tid; k < SO * V; k += 64) {
+      const int sl = k / V, v = k - sl * V;
+      sUo[k] = Ue[(sl == 0 ? 0 : supo[sl - 1]) * V + v];
+    }
+    if constexpr (NBR) {
+      const real* Un = a.Un + en * (NN * V);
+      for (int k = tid; k < SN * V; k += 64) {
+        const int sl = k / V, v = k - sl * V;
+        sUn[k] = Un[(sl == 0 ? 0 : supn[sl - 1]) * V + v];
+      }
+    }
+  }
   if (visc) {
-    for (int i = tid; i < NO; i += 64) viscous_variables<V, real>(sUo + i * V, gamma, ph.gas_constant, svo + i * 4);
+    for (int i = tid; i < NO; i += 64) {
+      real u[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) u[v] = a.Uo[eo * (NO * V) + i * V + v];
+      viscous_variables<5, real>(u, gamma, ph.gas_constant, svo + i * 4);
+    }
     if constexpr (NBR)
-      for (int i = tid; i < NN; i += 64) viscous_variables<V, real>(sUn + i * V, gamma, ph.gas_constant, svn + i * 4);
+      for (int i = tid; i < NN; i += 64) {
+        real u[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) u[v] = a.Un[en * (NN * V) + i * V + v];
+        viscous_variables<5, real>(u, gamma, ph.gas_constant, svn + i * 4);
+      }
   }
   __syncthreads();
 
@@ -548,7 +582,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
 #pragma unroll UO
     for (int kk = 0; kk < FO; ++kk) {
       const real b = phio[kk];
-      const real u = sUo[supo[kk] * V + v];
+      const real u = sUo[(kk + 1) * V + v];
       up += b * u;
       ua += b * (u - u0);
     }
@@ -561,7 +595,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
 #pragma unroll UN
       for (int kk = 0; kk < FN; ++kk) {
         const real b = phin[kk];
-        const real u = sUn[supn[kk] * V + v];
+        const real u = sUn[(kk + 1) * V + v];
         vp += b * u;
         ub += b * (u - n0);
       }
