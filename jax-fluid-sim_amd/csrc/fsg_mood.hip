@@ -152,6 +152,16 @@ __device__ __forceinline__ real wmax(real x) {
   return x;
 }
 __device__ __forceinline__ bool wany(bool b) { return __ballot(b) != 0ull; }
+// Wave maximum that ignores NaN (fmax): one instruction per step.  Used for
+// the deviation bounds of the sampling shortcuts, whose decision is false
+// anyway when an input is NaN or infinite (the mean, and with it the bounds,
+// are then NaN: every comparison with the floors fails).
+template <typename real>
+__device__ __forceinline__ real wmax_num(real x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
 
 template <typename real>
 __device__ __forceinline__ int locate(const Args<real>& a, int64_t g, int64_t* e) {
@@ -404,7 +414,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
 #pragma unroll
       for (int v = 0; v < V; ++v) fine = fine && isnum(sU[w][lane][v]) && fabs(sU[w][lane][v]) <= big;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) D[v] = wmax(lane < N ? fabs(sU[w][lane][v] - mv[v]) : real(0));
+    for (int v = 0; v < 5; ++v) D[v] = wmax_num(lane < N ? fabs(sU[w][lane][v] - mv[v]) : real(0));
     const real L = T.lebesgue;
     const real eps = sizeof(real) == 4 ? real(1.2e-7) : real(2.3e-16);
     const real rnd = real(16) * real(N + 2) * eps * L;
@@ -831,7 +841,7 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const real dev = lane < N ? fabs(sU[lane][v] - mean[v]) : real(0);
-      D[v] = wmax(dev);
+      D[v] = wmax_num(dev);
     }
     const real L = T.lebesgue;
     const real eps = sizeof(real) == 4 ? real(1.2e-7) : real(2.3e-16);
@@ -935,12 +945,13 @@ constexpr int MAXNT = 64 + 640;   // nodes + sample points of one element
 // mode 0: out = positivity(src); mode 1 (stage): out = positivity(alpha base
 // + beta positivity(work)) with the node admissibility reductions.
 template <int V, int KS, typename real>
-__global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real> ph, int mode) {
+__global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real> ph, int mode,
+                                                        int64_t g0) {
   __shared__ real sU[MAXN][V];
   __shared__ real sM[V];
   const int lane = threadIdx.x;
   int64_t e;
-  const int b = locate(a, (int64_t)blockIdx.x, &e);
+  const int b = locate(a, g0 + (int64_t)blockIdx.x, &e);
   const BlockT<real>& B = a.blocks[b];
   const TypeT<real>& T = a.types[B.type];
   const int N = T.N;
@@ -996,7 +1007,7 @@ struct fsg_mood {
   void* d_blocks = nullptr;
   void* d_groups = nullptr;
   std::vector<void*> d_mw;      // per-type node weights of the volume mean
-  int ks = 1;                   // samples per lane of the positivity kernel
+  std::vector<int> block_ks;    // samples per lane of the positivity kernel, per block
   std::vector<int64_t> offsets;
   std::vector<int32_t*> flags;
 };
@@ -1012,6 +1023,7 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
   std::vector<TypeT<real>> tt(3);
   memset(tt.data(), 0, sizeof(TypeT<real>) * 3);
   std::vector<bool> used(3, false);
+  int type_ks[3] = {1, 1, 1};
   for (const auto& b : c->blocks) used[b.element_type] = true;
   for (int t = 0; t < 3; ++t) {
     if (!used[t]) continue;
@@ -1079,7 +1091,7 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
       return fsg_fail(-21, "cannot upload the MOOD mean weights");
     d.mw = static_cast<const real*>(dmw);
     d.KS = (s.N + s.NS + 63) / 64;
-    if (d.KS > m->ks) m->ks = d.KS;
+    type_ks[t] = d.KS;
   }
   const int nb = (int)c->blocks.size();
   std::vector<BlockT<real>> bb(nb);
@@ -1097,6 +1109,7 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
     d.flags = blocks[b].flags;
     d.sensor = static_cast<real*>(blocks[b].sensor);
     m->offsets.push_back(off);
+    m->block_ks.push_back(type_ks[cb.element_type]);
     m->flags.push_back(blocks[b].flags);
     off += cb.count;
   }
@@ -1216,18 +1229,29 @@ int run_detect(fsg_mood* m, Args<real>& a, const Phys<real>& ph, hipStream_t s) 
   return launched("MOOD compaction");
 }
 
+// One launch per element block, each with the sample count per lane of its
+// element type (hexahedra need 2 at p = 2 where tetrahedra need 8: the
+// deviation registers, and with them the occupancy, follow the block).
 template <int V, typename real>
 void launch_positivity(fsg_mood* m, const Args<real>& a, const Phys<real>& ph, int mode,
                        hipStream_t s) {
-  const dim3 g((unsigned)m->total), b(64);
-  if (m->ks <= 4)
-    hipLaunchKernelGGL((positivity_kernel<V, 4, real>), g, b, 0, s, a, ph, mode);
-  else if (m->ks <= 5)
-    hipLaunchKernelGGL((positivity_kernel<V, 5, real>), g, b, 0, s, a, ph, mode);
-  else if (m->ks <= 8)
-    hipLaunchKernelGGL((positivity_kernel<V, 8, real>), g, b, 0, s, a, ph, mode);
-  else
-    hipLaunchKernelGGL((positivity_kernel<V, 11, real>), g, b, 0, s, a, ph, mode);
+  for (size_t k = 0; k < m->offsets.size(); ++k) {
+    const int64_t count = (k + 1 < m->offsets.size() ? m->offsets[k + 1] : m->total) - m->offsets[k];
+    if (count <= 0) continue;
+    const dim3 g((unsigned)count), b(64);
+    const int64_t g0 = m->offsets[k];
+    const int ks = m->block_ks[k];
+    if (ks <= 2)
+      hipLaunchKernelGGL((positivity_kernel<V, 2, real>), g, b, 0, s, a, ph, mode, g0);
+    else if (ks <= 4)
+      hipLaunchKernelGGL((positivity_kernel<V, 4, real>), g, b, 0, s, a, ph, mode, g0);
+    else if (ks <= 5)
+      hipLaunchKernelGGL((positivity_kernel<V, 5, real>), g, b, 0, s, a, ph, mode, g0);
+    else if (ks <= 8)
+      hipLaunchKernelGGL((positivity_kernel<V, 8, real>), g, b, 0, s, a, ph, mode, g0);
+    else
+      hipLaunchKernelGGL((positivity_kernel<V, 11, real>), g, b, 0, s, a, ph, mode, g0);
+  }
 }
 
 template <int V, typename real>
