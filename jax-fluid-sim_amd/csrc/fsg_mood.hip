@@ -375,6 +375,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     real hi0 = 0, hi1 = 0, sc0 = 0, sc1 = 0;
     if (lane < N) {
       real l0 = 0, l1 = 0;
+#pragma unroll 8
       for (int j = 0; j < N; ++j) {
         const real c = T.p1proj[j * N + lane];   // transposed table: P1[lane][j], coalesced
         l0 += c * sm[w][j][0];
@@ -404,6 +405,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
   {
     real mpart = real(0);
     if (lane < 5)
+#pragma unroll 8
       for (int j = 0; j < N; ++j) mpart += T.mw[j] * sU[w][j][lane];
     real mv[5], D[5];
 #pragma unroll
@@ -815,6 +817,7 @@ __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const
   const int N = T.N, NT = N + T.NS;
   if (lane < V) {
     real m = real(0);
+#pragma unroll 8
     for (int j = 0; j < N; ++j) m += T.mw[j] * sU[j][lane];
     sM[lane] = m / T.wsum;
   }
