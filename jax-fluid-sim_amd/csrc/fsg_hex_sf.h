@@ -75,14 +75,18 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   constexpr int SX = N1 * N1 * Q1 * CX, SY = N1 * Q1 * Q1 * CY, SZ = Q * CZ;
   constexpr int SF = Q * 3 * V, STZ = N1 * Q1 * Q1 * 3 * V, STY = N1 * N1 * Q1 * 2 * V;
   constexpr int W1S = cmax(cmax(SX, SZ), cmax(STZ, N * 12));
-  constexpr int W2S = cmax(cmax(SY, SF), cmax(STY, N * 12));
+  // w2 also holds the lift right-hand side / lift (sb) and the viscous
+  // variables (sv) until the x pass of the interpolation has read them: they
+  // sit behind the first N * 12 entries, which the inverse-mass passes of
+  // the lift use as their middle buffer; the y pass overwrites them later.
+  constexpr int W2S = cmax(cmax(cmax(SY, SF), cmax(STY, N * 12)), N * 28);
 
   __shared__ real sU[N * V];
   __shared__ real sR[N * V];
-  __shared__ real sb[N * 12];
-  __shared__ real sv[N * 4];
   __shared__ real w1[W1S];
   __shared__ real w2[W2S];
+  real* const sb = w2 + N * 12;      // [N][12], dead after the x pass
+  real* const sv = w2 + N * 24;      // [N][4], dead after the x pass
   __shared__ real sB1[Q1 * N1], sD1[Q1 * N1], sM1[N1 * N1];
   __shared__ double red[2][NT / 64];
   __shared__ unsigned int redn[NT / 64];
