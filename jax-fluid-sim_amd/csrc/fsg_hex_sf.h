@@ -62,6 +62,21 @@ __device__ __forceinline__ void pencil(const real* op, const real* in, int si, r
   }
 }
 
+// The same line contraction with the outputs kept in registers.
+template <int NO, int NI, typename real>
+__device__ __forceinline__ void pencil_vals(const real* op, const real* in, int si, real* out) {
+  real x[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) x[i] = in[i * si];
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    real acc = real(0);
+#pragma unroll
+    for (int i = 0; i < NI; ++i) acc += op[o * NI + i] * x[i];
+    out[o] = acc;
+  }
+}
+
 template <int P, int V, typename real, int NT>
 __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real> ph) {
   using S = HexSF<P>;
@@ -71,7 +86,7 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   constexpr int CI = V + 16;        // x pass with B1: U - a (V), lift (12), v (4)
   constexpr int CX = CI + 4;        // ... plus D1 along x of v (4)
   constexpr int CY = CI + 8;        // y pass: B1 of CI, D1 of v (d/dy), B1 of the x derivative
-  constexpr int CZ = V + 24;        // z pass: U - a, lift, dv/dx, dv/dy, dv/dz
+  constexpr int CZ = V + 12;        // z pass: U - a, g = J^-T grad v + lift
   constexpr int SX = N1 * N1 * Q1 * CX, SY = N1 * Q1 * Q1 * CY, SZ = Q * CZ;
   constexpr int SF = Q * 3 * V, STZ = N1 * Q1 * Q1 * 3 * V, STY = N1 * N1 * Q1 * 2 * V;
   constexpr int W1S = cmax(cmax(SX, SZ), cmax(STZ, N * 12));
@@ -208,19 +223,32 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
                                     w2 + (k * Q1 * Q1 + qa) * CY + CI + m, Q1 * CY);
       }
     __syncthreads();
-    // z pass: (k, b, a) -> (c, b, a); B1 except D1 for d/dz
-    constexpr int NBZ = VI ? V + 20 : V;
-    for (int t = tid; t < Q1 * Q1 * NBZ; t += NT) {
-      const int ab = t / NBZ, c = t - ab * NBZ;
-      const int src = c < V + 12 ? c : (c < V + 16 ? CI + 4 + (c - V - 12) : CI + (c - V - 16));
-      pencil<Q1, N1, false, real>(sB1, w2 + ab * CY + src, Q1 * Q1 * CY, w1 + ab * CZ + c,
+    // z pass: (k, b, a) -> (c, b, a); B1 on U - a.  With the viscous terms,
+    // one thread per (line, m) also takes the lift (B1), dv_m/dx and dv_m/dy
+    // (B1) and dv_m/dz (D1) along the line and stores the physical gradient
+    // plus lift g = r + J^-T grad v per point, 12 columns instead of 24.
+    for (int t = tid; t < Q1 * Q1 * V; t += NT) {
+      const int ab = t / V, c = t - ab * V;
+      pencil<Q1, N1, false, real>(sB1, w2 + ab * CY + c, Q1 * Q1 * CY, w1 + ab * CZ + c,
                                   Q1 * Q1 * CZ);
     }
     if constexpr (VI)
       for (int t = tid; t < Q1 * Q1 * 4; t += NT) {
         const int ab = t >> 2, m = t & 3;
-        pencil<Q1, N1, false, real>(sD1, w2 + ab * CY + V + 12 + m, Q1 * Q1 * CY,
-                                    w1 + ab * CZ + V + 20 + m, Q1 * Q1 * CZ);
+        const real* in = w2 + ab * CY;
+        real lf[3][Q1], gx[Q1], gy[Q1], gz[Q1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          pencil_vals<Q1, N1, real>(sB1, in + V + m * 3 + d, Q1 * Q1 * CY, lf[d]);
+        pencil_vals<Q1, N1, real>(sB1, in + CI + 4 + m, Q1 * Q1 * CY, gx);
+        pencil_vals<Q1, N1, real>(sB1, in + CI + m, Q1 * Q1 * CY, gy);
+        pencil_vals<Q1, N1, real>(sD1, in + V + 12 + m, Q1 * Q1 * CY, gz);
+#pragma unroll
+        for (int c = 0; c < Q1; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            w1[(c * Q1 * Q1 + ab) * CZ + V + m * 3 + d] =
+                lf[d][c] + (gx[c] * J[0 * 3 + d] + gy[c] * J[1 * 3 + d] + gz[c] * J[2 * 3 + d]);
       }
     __syncthreads();
   };
@@ -253,12 +281,7 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
     if (visc) {
       real g[12];
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          g[m * 3 + d] = z[V + m * 3 + d] + (z[V + 12 + m] * J[0 * 3 + d] +
-                                             z[V + 16 + m] * J[1 * 3 + d] +
-                                             z[V + 20 + m] * J[2 * 3 + d]);
+      for (int k = 0; k < 12; ++k) g[k] = z[V + k];
       real Fv[3 * V];
       viscous_flux<V, real>(Ua, g, gamma, ph.mu, ph.kappa, Fv);
 #pragma unroll
