@@ -90,18 +90,21 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   constexpr int SX = N1 * N1 * Q1 * CX, SY = N1 * Q1 * Q1 * CY, SZ = Q * CZ;
   constexpr int SF = Q * 3 * V, STZ = N1 * Q1 * Q1 * 3 * V, STY = N1 * N1 * Q1 * 2 * V;
   constexpr int W1S = cmax(cmax(SX, SZ), cmax(STZ, N * 12));
-  // w2 also holds the lift right-hand side / lift (sb) and the viscous
-  // variables (sv) until the x pass of the interpolation has read them: they
-  // sit behind the first N * 12 entries, which the inverse-mass passes of
-  // the lift use as their middle buffer; the y pass overwrites them later.
-  constexpr int W2S = cmax(cmax(cmax(SY, SF), cmax(STY, N * 12)), N * 28);
+  // w2 also holds the lift right-hand side / lift (sb), the viscous
+  // variables (sv) and the element state (sU) until the x pass of the
+  // interpolation has read them: they sit behind the first N * 12 entries,
+  // which the inverse-mass passes of the lift use as their middle buffer;
+  // the y pass overwrites them later.  The anchor (node 0) is kept apart and
+  // the stage epilogue reads the state again from memory.
+  constexpr int W2S = cmax(cmax(cmax(SY, SF), cmax(STY, N * 12)), N * (28 + V));
 
-  __shared__ real sU[N * V];
   __shared__ real sR[N * V];
+  __shared__ real sA[V];
   __shared__ real w1[W1S];
   __shared__ real w2[W2S];
   real* const sb = w2 + N * 12;      // [N][12], dead after the x pass
   real* const sv = w2 + N * 24;      // [N][4], dead after the x pass
+  real* const sU = w2 + N * 28;      // [N][V], dead after the x pass
   __shared__ real sB1[Q1 * N1], sD1[Q1 * N1], sM1[N1 * N1];
   __shared__ double red[2][NT / 64];
   __shared__ unsigned int redn[NT / 64];
@@ -116,6 +119,7 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   const real* Ue = a.U + e * (N * V);
   for (int k = tid; k < N * V; k += NT) {
     sU[k] = Ue[k];
+    if (k < V) sA[k] = Ue[k];
     sR[k] = real(0);
   }
   for (int k = tid; k < N * 12; k += NT) sb[k] = real(0);
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
       const real* src;
       int stride;
       real sub = real(0);
-      if (c < V) { src = sU + c; stride = V; sub = sU[c]; }
+      if (c < V) { src = sU + c; stride = V; sub = sA[c]; }
       else if (c < V + 12) { src = sb + (c - V); stride = 12; }
       else { src = sv + (c - V - 12); stride = 4; }
       pencil<Q1, N1, false, real>(sB1, src + jk * N1 * stride, stride, w1 + jk * Q1 * CX + c, CX,
@@ -262,18 +266,18 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
     const real* z = w1 + q * CZ;
     real Ua[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) Ua[v] = z[v] + sU[v];
+    for (int v = 0; v < V; ++v) Ua[v] = z[v] + sA[v];
     real dF[3 * V];
 #pragma unroll
     for (int k = 0; k < 3 * V; ++k) dF[k] = real(0);
     if (euler) {
       Prim<V, real> wq = c2p<V, real>(Ua, gamma);
-      Prim<V, real> wa = c2p<V, real>(sU, gamma);
+      Prim<V, real> wa = c2p<V, real>(sA, gamma);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         real Fq[V], Fa[V];
         axis_flux<V, real>(Ua, wq, d, Fq);
-        axis_flux<V, real>(sU, wa, d, Fa);
+        axis_flux<V, real>(sA, wa, d, Fa);
 #pragma unroll
         for (int v = 0; v < V; ++v) dF[d * V + v] = Fq[v] - Fa[v];
       }
@@ -370,7 +374,7 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
     if (a.mode == 1) {
       out[k] = rhs;
     } else {
-      const real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (sU[k] + dte * rhs);
+      const real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (Ue[k] + dte * rhs);
       out[k] = nxt;
       sR[k] = nxt;
     }
