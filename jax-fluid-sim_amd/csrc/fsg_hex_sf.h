@@ -89,14 +89,17 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   constexpr int CZ = V + 12;        // z pass: U - a, g = J^-T grad v + lift
   constexpr int SX = N1 * N1 * Q1 * CX, SY = N1 * Q1 * Q1 * CY, SZ = Q * CZ;
   constexpr int SF = Q * 3 * V, STZ = N1 * Q1 * Q1 * 3 * V, STY = N1 * N1 * Q1 * 2 * V;
-  constexpr int W1S = cmax(cmax(SX, SZ), cmax(STZ, N * 12));
+  // the transposed z pass runs in place in w2 (each thread reads its whole
+  // line before writing the N1 <= Q1 results back into it); the transposed
+  // y pass writes w1
+  constexpr int W1S = cmax(cmax(SX, SZ), cmax(STY, N * 12));
   // w2 also holds the lift right-hand side / lift (sb), the viscous
   // variables (sv) and the element state (sU) until the x pass of the
   // interpolation has read them: they sit behind the first N * 12 entries,
   // which the inverse-mass passes of the lift use as their middle buffer;
   // the y pass overwrites them later.  The anchor (node 0) is kept apart and
   // the stage epilogue reads the state again from memory.
-  constexpr int W2S = cmax(cmax(cmax(SY, SF), cmax(STY, N * 12)), N * (28 + V));
+  constexpr int W2S = cmax(cmax(cmax(SY, SF), cmax(STZ, N * 12)), N * (28 + V));
 
   __shared__ real sR[N * V];
   __shared__ real sA[V];
@@ -306,12 +309,12 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   // z: (c, b, a) -> (k, b, a), D1^T for r = z, B1^T otherwise
   for (int t = tid; t < Q1 * Q1 * 2 * V; t += NT) {
     const int ab = t / (2 * V), col = t - ab * (2 * V);
-    pencil<N1, Q1, true, real>(sB1, w2 + ab * 3 * V + col, Q1 * Q1 * 3 * V, w1 + ab * 3 * V + col,
+    pencil<N1, Q1, true, real>(sB1, w2 + ab * 3 * V + col, Q1 * Q1 * 3 * V, w2 + ab * 3 * V + col,
                                Q1 * Q1 * 3 * V);
   }
   for (int t = tid; t < Q1 * Q1 * V; t += NT) {
     const int ab = t / V, col = 2 * V + (t - ab * V);
-    pencil<N1, Q1, true, real>(sD1, w2 + ab * 3 * V + col, Q1 * Q1 * 3 * V, w1 + ab * 3 * V + col,
+    pencil<N1, Q1, true, real>(sD1, w2 + ab * 3 * V + col, Q1 * Q1 * 3 * V, w2 + ab * 3 * V + col,
                                Q1 * Q1 * 3 * V);
   }
   __syncthreads();
@@ -319,14 +322,14 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   // r = z (both taken by B1^T in x next); [V, 2V): B1^T of r = x (D1^T in x)
   for (int t = tid; t < N1 * Q1 * V; t += NT) {
     const int ka = t / V, v = t - ka * V, k = ka / Q1, qa = ka - k * Q1;
-    const real* in = w1 + (k * Q1 * Q1 + qa) * 3 * V;
+    const real* in = w2 + (k * Q1 * Q1 + qa) * 3 * V;
     real ry[Q1], rz[Q1];
 #pragma unroll
     for (int qb = 0; qb < Q1; ++qb) {
       ry[qb] = in[qb * Q1 * 3 * V + V + v];
       rz[qb] = in[qb * Q1 * 3 * V + 2 * V + v];
     }
-    real* o = w2 + (k * N1 * Q1 + qa) * 2 * V + v;
+    real* o = w1 + (k * N1 * Q1 + qa) * 2 * V + v;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       real acc = real(0);
@@ -337,14 +340,14 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   }
   for (int t = tid; t < N1 * Q1 * V; t += NT) {
     const int ka = t / V, v = t - ka * V, k = ka / Q1, qa = ka - k * Q1;
-    pencil<N1, Q1, true, real>(sB1, w1 + (k * Q1 * Q1 + qa) * 3 * V + v, Q1 * 3 * V,
-                               w2 + (k * N1 * Q1 + qa) * 2 * V + V + v, Q1 * 2 * V);
+    pencil<N1, Q1, true, real>(sB1, w2 + (k * Q1 * Q1 + qa) * 3 * V + v, Q1 * 3 * V,
+                               w1 + (k * N1 * Q1 + qa) * 2 * V + V + v, Q1 * 2 * V);
   }
   __syncthreads();
   // x: (k, j, a) -> (k, j, i), added to the surface terms
   for (int t = tid; t < N1 * N1 * V; t += NT) {
     const int jk = t / V, v = t - jk * V;
-    const real* in = w2 + jk * Q1 * 2 * V;
+    const real* in = w1 + jk * Q1 * 2 * V;
     real xb[Q1], xd[Q1];
 #pragma unroll
     for (int qa = 0; qa < Q1; ++qa) {
