@@ -265,43 +265,57 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
     interpolate(std::false_type());
 
   // Pointwise fluxes, pulled back to reference directions with the weight.
+  // dF holds the five flow columns; a species column's Euler flux is U_s u_d
+  // (its viscous flux is zero), so its pulled-back difference is formed from
+  // the contravariant velocities.
+  constexpr int VF = 5;
   for (int q = tid; q < Q; q += NT) {
     const real* z = w1 + q * CZ;
     real Ua[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) Ua[v] = z[v] + sA[v];
-    real dF[3 * V];
+    real dF[3 * VF];
 #pragma unroll
-    for (int k = 0; k < 3 * V; ++k) dF[k] = real(0);
+    for (int k = 0; k < 3 * VF; ++k) dF[k] = real(0);
+    real uq[3] = {0, 0, 0}, ua[3] = {0, 0, 0};
     if (euler) {
-      Prim<V, real> wq = c2p<V, real>(Ua, gamma);
-      Prim<V, real> wa = c2p<V, real>(sA, gamma);
+      Prim<VF, real> wq = c2p<VF, real>(Ua, gamma);
+      Prim<VF, real> wa = c2p<VF, real>(sA, gamma);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        real Fq[V], Fa[V];
-        axis_flux<V, real>(Ua, wq, d, Fq);
-        axis_flux<V, real>(sA, wa, d, Fa);
+        real Fq[VF], Fa[VF];
+        axis_flux<VF, real>(Ua, wq, d, Fq);
+        axis_flux<VF, real>(sA, wa, d, Fa);
 #pragma unroll
-        for (int v = 0; v < V; ++v) dF[d * V + v] = Fq[v] - Fa[v];
+        for (int v = 0; v < VF; ++v) dF[d * VF + v] = Fq[v] - Fa[v];
+        uq[d] = wq.vel[d];
+        ua[d] = wa.vel[d];
       }
     }
     if (visc) {
       real g[12];
 #pragma unroll
       for (int k = 0; k < 12; ++k) g[k] = z[V + k];
-      real Fv[3 * V];
-      viscous_flux<V, real>(Ua, g, gamma, ph.mu, ph.kappa, Fv);
+      real Fv[3 * VF];
+      viscous_flux<VF, real>(Ua, g, gamma, ph.mu, ph.kappa, Fv);
 #pragma unroll
-      for (int k = 0; k < 3 * V; ++k) dF[k] -= Fv[k];
+      for (int k = 0; k < 3 * VF; ++k) dF[k] -= Fv[k];
     }
     const real w = a.wq[q] * dw;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r) {
 #pragma unroll
-      for (int v = 0; v < V; ++v)
+      for (int v = 0; v < VF; ++v)
         w2[q * 3 * V + r * V + v] =
-            w * (J[r * 3 + 0] * dF[0 * V + v] + J[r * 3 + 1] * dF[1 * V + v] +
-                 J[r * 3 + 2] * dF[2 * V + v]);
+            w * (J[r * 3 + 0] * dF[0 * VF + v] + J[r * 3 + 1] * dF[1 * VF + v] +
+                 J[r * 3 + 2] * dF[2 * VF + v]);
+      if constexpr (V > VF) {
+        const real ur = J[r * 3 + 0] * uq[0] + J[r * 3 + 1] * uq[1] + J[r * 3 + 2] * uq[2];
+        const real uar = J[r * 3 + 0] * ua[0] + J[r * 3 + 1] * ua[1] + J[r * 3 + 2] * ua[2];
+#pragma unroll
+        for (int v = VF; v < V; ++v) w2[q * 3 * V + r * V + v] = w * (Ua[v] * ur - sA[v] * uar);
+      }
+    }
   }
   __syncthreads();
 

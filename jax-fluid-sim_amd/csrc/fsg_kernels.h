@@ -193,39 +193,46 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   }
   __syncthreads();
 
-  // Volume quadrature points.
+  // Volume quadrature points.  The flux differences dF are held for the five
+  // flow columns only; a species column's Euler flux is U_s u_d, so its
+  // pulled-back difference is formed from the contravariant velocities at
+  // the end, and its viscous flux is zero (the plain interpolant Up, used by
+  // the viscous flux, needs the flow columns only).
+  constexpr int VF = 5;
   for (int q = tid; q < Q; q += NT) {
-    real dF[3 * V];
+    real dF[3 * VF];
 #pragma unroll
-    for (int k = 0; k < 3 * V; ++k) dF[k] = real(0);
-    real Ua[V], Up[V];
+    for (int k = 0; k < 3 * VF; ++k) dF[k] = real(0);
+    real Ua[V], Up[VF];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      Ua[v] = real(0);
-      Up[v] = real(0);
-    }
+    for (int v = 0; v < V; ++v) Ua[v] = real(0);
+#pragma unroll
+    for (int v = 0; v < VF; ++v) Up[v] = real(0);
     for (int i = 0; i < N; ++i) {
       real b = a.Bt[i * Q + q];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         real u = sU[i * V + v];
-        Up[v] += b * u;
+        if (v < VF) Up[v] += b * u;
         Ua[v] += b * (u - sU[v]);
       }
     }
+    real uq[3] = {0, 0, 0}, ua[3] = {0, 0, 0};
     if (euler) {
       // Anchored interpolation and baseline flux (hybrid/spatial.py:41-70).
 #pragma unroll
       for (int v = 0; v < V; ++v) Ua[v] += sU[v];
-      Prim<V, real> wq = c2p<V, real>(Ua, gamma);
-      Prim<V, real> wa = c2p<V, real>(sU, gamma);
+      Prim<VF, real> wq = c2p<VF, real>(Ua, gamma);
+      Prim<VF, real> wa = c2p<VF, real>(sU, gamma);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        real Fq[V], Fa[V];
-        axis_flux<V, real>(Ua, wq, d, Fq);
-        axis_flux<V, real>(sU, wa, d, Fa);
+        real Fq[VF], Fa[VF];
+        axis_flux<VF, real>(Ua, wq, d, Fq);
+        axis_flux<VF, real>(sU, wa, d, Fa);
 #pragma unroll
-        for (int v = 0; v < V; ++v) dF[d * V + v] = Fq[v] - Fa[v];
+        for (int v = 0; v < VF; ++v) dF[d * VF + v] = Fq[v] - Fa[v];
+        uq[d] = wq.vel[d];
+        ua[d] = wa.vel[d];
       }
     }
     if (visc) {
@@ -256,20 +263,27 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
         for (int d = 0; d < 3; ++d)
           g[m * 3 + d] += G[m * 3 + 0] * J[0 * 3 + d] + G[m * 3 + 1] * J[1 * 3 + d] +
                           G[m * 3 + 2] * J[2 * 3 + d];
-      real Fv[3 * V];
-      viscous_flux<V, real>(Up, g, gamma, ph.mu, ph.kappa, Fv);
+      real Fv[3 * VF];
+      viscous_flux<VF, real>(Up, g, gamma, ph.mu, ph.kappa, Fv);
 #pragma unroll
-      for (int k = 0; k < 3 * V; ++k) dF[k] -= Fv[k];
+      for (int k = 0; k < 3 * VF; ++k) dF[k] -= Fv[k];
     }
     // Pull back to reference directions with the quadrature weight.
     const real w = a.wq[q] * dw;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r) {
 #pragma unroll
-      for (int v = 0; v < V; ++v)
+      for (int v = 0; v < VF; ++v)
         sF[(q * 3 + r) * V + v] =
-            w * (J[r * 3 + 0] * dF[0 * V + v] + J[r * 3 + 1] * dF[1 * V + v] +
-                 J[r * 3 + 2] * dF[2 * V + v]);
+            w * (J[r * 3 + 0] * dF[0 * VF + v] + J[r * 3 + 1] * dF[1 * VF + v] +
+                 J[r * 3 + 2] * dF[2 * VF + v]);
+      if constexpr (V > VF) {
+        const real ur = J[r * 3 + 0] * uq[0] + J[r * 3 + 1] * uq[1] + J[r * 3 + 2] * uq[2];
+        const real uar = J[r * 3 + 0] * ua[0] + J[r * 3 + 1] * ua[1] + J[r * 3 + 2] * ua[2];
+#pragma unroll
+        for (int v = VF; v < V; ++v) sF[(q * 3 + r) * V + v] = w * (Ua[v] * ur - sU[v] * uar);
+      }
+    }
   }
   __syncthreads();
 
@@ -963,11 +977,13 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
 #pragma unroll
         for (int k = 0; k < 3 * V; ++k) Fo[k] = real(0.5) * (Fo[k] + Fn[k]);
       }
+      // the viscous flux has momentum and energy rows only (the mass and
+      // species rows are zero): the other columns are left as they are
 #pragma unroll
-      for (int v = 0; v < V; ++v) fn[v] = Fo[v] * n[0] + Fo[V + v] * n[1] + Fo[2 * V + v] * n[2];
+      for (int v = 1; v < 5; ++v) fn[v] = Fo[v] * n[0] + Fo[V + v] * n[1] + Fo[2 * V + v] * n[2];
       if (!NBR && ph.has_boundary && (flags & BND_WALL)) fn[4] = real(0);
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
+      for (int v = 1; v < 5; ++v) {
         sSo[q * V + v] += w * fn[v];
         sSn[q * V + v] -= w * fn[v];
       }
