@@ -456,9 +456,10 @@ __global__ __launch_bounds__(64) void finish_kernel(ureal* __restrict__ U, int64
 
 // Integrator on the flagged cells of a batch only (the surrogate's
 // fallback), without a host round trip: flagged cell ids are compacted with
-// wavefront ballots into list[0..*count), then a launch of one thread per
-// batch cell integrates list entries; threads past *count return at once.  Cells are independent, so the list order (set by the atomics) does
-// not change any result.
+// wavefront ballots into list[0..*count), then a bounded grid strides over
+// the list entries (and returns at once when the list is empty).  Cells are
+// independent, so the list order (set by the atomics) does not change any
+// result.
 __global__ void zero_count_kernel(int32_t* count) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0;
 }
@@ -554,8 +555,11 @@ int fsg_chem_integrate_flagged(const double* table, const double* rho_in, const 
   hipLaunchKernelGGL(fsgchem::zero_count_kernel, dim3(1), dim3(64), 0, s, count);
   hipLaunchKernelGGL(fsgchem::compact_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
                      0, s, flag, n, list, count);
-  // one thread per possible list entry: blocks past *count return at once
-  hipLaunchKernelGGL(fsgchem::integrate_list_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64),
+  // a bounded grid strides over the list: 8192 64-lane workgroups exceed the
+  // integrator's resident capacity (one wave per SIMD at its register count)
+  // and leave at once when nothing is flagged
+  const int64_t blocks = (n + 63) / 64 < 8192 ? (n + 63) / 64 : 8192;
+  hipLaunchKernelGGL(fsgchem::integrate_list_kernel, dim3((unsigned)blocks), dim3(64),
                      0, s, table,
                      rho_in, T_in, dt_cell, dt, newton_iters, constant_temperature, list, count,
                      rho_out, T_out);
