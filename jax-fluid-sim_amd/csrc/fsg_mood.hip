@@ -1219,7 +1219,7 @@ int launched(const char* what) {
   return e == hipSuccess ? 0 : fsg_fail(-1, "%s launch failed: %s", what, hipGetErrorString(e));
 }
 
-constexpr unsigned SUBCELL_GRID = 2048;
+constexpr unsigned SUBCELL_GRID = 8192;
 
 // detect (+ compaction)
 template <int V, typename real>
