@@ -676,13 +676,20 @@ __global__ __launch_bounds__(64) void subcell_kernel(Args<real> a, Phys<real> ph
         const GroupT<real>& G = a.groups[code & 0xffff];
         const int64_t i = B.elem_faces[(e * B.max_faces + fe) * 2 + 1];
         const int tab = G.pair_table[i];
-        for (int kk = 0; kk < G.K; ++kk) {
-          if (G.pair_ids[((int64_t)tab * G.K + kk) * 2 + side] != j) continue;
+        const int32_t* ids = G.pair_ids + (int64_t)tab * G.K * 2 + side;
+        // Each lane walks its own matching pairs (in increasing order); the
+        // flux evaluations of all lanes then run together, one round per
+        // match rather than one per pair index any lane matches.
+        int kk = 0;
+        while (true) {
+          while (kk < G.K && ids[kk * 2] != j) ++kk;
+          if (kk >= G.K) break;
           real F[V];
           pair_flux<V, real>(a, ph, G, i, kk, F);
           const real sg = side == 0 ? real(-1) : real(1);
 #pragma unroll
           for (int v = 0; v < V; ++v) res[v] += sg * F[v];
+          ++kk;
         }
       }
     }
