@@ -208,6 +208,7 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
     for (int v = 0; v < V; ++v) Ua[v] = real(0);
 #pragma unroll
     for (int v = 0; v < VF; ++v) Up[v] = real(0);
+#pragma unroll 2
     for (int i = 0; i < N; ++i) {
       real b = a.Bt[i * Q + q];
 #pragma unroll
@@ -242,6 +243,7 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
         G[k] = real(0);
         g[k] = real(0);
       }
+#pragma unroll 2
       for (int i = 0; i < N; ++i) {
         real dx = a.Dt[(i * 3 + 0) * Q + q];
         real dy = a.Dt[(i * 3 + 1) * Q + q];
