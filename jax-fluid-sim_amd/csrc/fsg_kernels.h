@@ -128,7 +128,10 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   __shared__ real sv[N * 4];
   __shared__ real sb[N * 12];
   __shared__ real sL[N * 12];
-  __shared__ real sF[Q * 3 * V];
+  // per point: the pulled-back flow fluxes [3][5]; with species also w (J_r .
+  // u) [3], w (J_r . u_a) [3] and the species interpolants [V - 5]
+  constexpr int SW = V > 5 ? 15 + 6 + (V - 5) : 3 * V;
+  __shared__ real sF[Q * SW];
   __shared__ double red[2][NT / 64];
   __shared__ unsigned int redn[NT / 64];
 
@@ -272,48 +275,66 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
     }
     // Pull back to reference directions with the quadrature weight.
     const real w = a.wq[q] * dw;
+    real* f = sF + q * SW;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
       for (int v = 0; v < VF; ++v)
-        sF[(q * 3 + r) * V + v] =
-            w * (J[r * 3 + 0] * dF[0 * VF + v] + J[r * 3 + 1] * dF[1 * VF + v] +
-                 J[r * 3 + 2] * dF[2 * VF + v]);
+        f[r * VF + v] = w * (J[r * 3 + 0] * dF[0 * VF + v] + J[r * 3 + 1] * dF[1 * VF + v] +
+                             J[r * 3 + 2] * dF[2 * VF + v]);
       if constexpr (V > VF) {
-        const real ur = J[r * 3 + 0] * uq[0] + J[r * 3 + 1] * uq[1] + J[r * 3 + 2] * uq[2];
-        const real uar = J[r * 3 + 0] * ua[0] + J[r * 3 + 1] * ua[1] + J[r * 3 + 2] * ua[2];
-#pragma unroll
-        for (int v = VF; v < V; ++v) sF[(q * 3 + r) * V + v] = w * (Ua[v] * ur - sU[v] * uar);
+        f[15 + r] = w * (J[r * 3 + 0] * uq[0] + J[r * 3 + 1] * uq[1] + J[r * 3 + 2] * uq[2]);
+        f[18 + r] = w * (J[r * 3 + 0] * ua[0] + J[r * 3 + 1] * ua[1] + J[r * 3 + 2] * ua[2]);
       }
     }
+    if constexpr (V > VF)
+#pragma unroll
+      for (int v = VF; v < V; ++v) f[21 + (v - VF)] = Ua[v];
   }
   __syncthreads();
 
   // R_i += sum_q sum_r dphi_i/dxi_r F^_r: DS adjacent lanes per node (DS a
   // power of two with N DS <= NT: 2 for 64 nodes, 8 for the 10-node
   // tetrahedron), each over every DS-th point and all variables (one read of
-  // D per point and direction), partial sums combined by lane shuffles.
+  // D per point and direction), partial sums combined by lane shuffles.  A
+  // species column's flux difference w (U_s (J_r . u) - a_s (J_r . u_a)) is
+  // rank two: its divergence is sum_q c_q U_s(q) - a_s sum_q c'_q with
+  // c_q = sum_r D_qri w (J_r . u), c'_q likewise with the anchor velocity.
   {
     constexpr int DS = div_split(N, NT);
     const int i = tid / DS, h = tid - i * DS;
-    real acc[V];
+    real acc[V], acc_a = real(0);
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = real(0);
     if (i < N)
-      for (int q = h; q < Q; q += DS)
+      for (int q = h; q < Q; q += DS) {
+        const real* f = sF + q * SW;
+        real d[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          const real d = a.D[(q * 3 + r) * N + i];
+          d[r] = a.D[(q * 3 + r) * N + i];
 #pragma unroll
-          for (int v = 0; v < V; ++v) acc[v] += d * sF[(q * 3 + r) * V + v];
+          for (int v = 0; v < VF; ++v) acc[v] += d[r] * f[r * VF + v];
         }
+        if constexpr (V > VF) {
+          const real c = d[0] * f[15] + d[1] * f[16] + d[2] * f[17];
+          acc_a += d[0] * f[18] + d[1] * f[19] + d[2] * f[20];
 #pragma unroll
-    for (int o = 1; o < DS; o <<= 1)
+          for (int v = VF; v < V; ++v) acc[v] += c * f[21 + (v - VF)];
+        }
+      }
+#pragma unroll
+    for (int o = 1; o < DS; o <<= 1) {
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], o, 64);
-    if (i < N && h == 0)
+      if constexpr (V > VF) acc_a += __shfl_xor(acc_a, o, 64);
+    }
+    if (i < N && h == 0) {
 #pragma unroll
-      for (int v = 0; v < V; ++v) sR[i * V + v] += acc[v];
+      for (int v = 0; v < VF; ++v) sR[i * V + v] += acc[v];
+#pragma unroll
+      for (int v = VF; v < V; ++v) sR[i * V + v] += acc[v] - sU[v] * acc_a;
+    }
   }
   __syncthreads();
 
