@@ -704,34 +704,10 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
           sSn[q * V + v] = real(0);
         }
       }
-      if (visc) {
-        const real* phio = tbo + q * FO;
-        real vo[4] = {0, 0, 0, 0}, vn[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < FO; ++k) {
-          real b = phio[k];
-#pragma unroll
-          for (int m = 0; m < 4; ++m) vo[m] += b * svo[supo[k] * 4 + m];
-        }
-        if constexpr (NBR) {
-          const real* phin = tbn + q * FN;
-#pragma unroll
-          for (int k = 0; k < FN; ++k) {
-            real b = phin[k];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) vn[m] += b * svn[supn[k] * 4 + m];
-          }
-        } else {
-          ghost_viscous<V, real>(sPo + q * V, n, flags, ph, vn);
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m) sJ[q * 4 + m] = w * (vn[m] - vo[m]);
-      }
     }
   } else
   for (int q = tid; q < QF; q += 64) {
     const real w = a.fw[q] * area;
-    const real* phio = tbo + q * FO;
     real Ua_o[V], Ua_n[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) Ua_o[v] = sAo[q * V + v];
@@ -768,27 +744,27 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
       sSo[q * V + v] = So[v];
       sSn[q * V + v] = Sn[v];
     }
-    if (visc) {
-      real vo[4] = {0, 0, 0, 0}, vn[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int k = 0; k < FO; ++k) {
-        real b = phio[k];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) vo[m] += b * svo[supo[k] * 4 + m];
-      }
+  }
+  // Jump of the viscous variables at the face points, one lane per (point,
+  // variable): [[v]]_m = v_m(neighbour trace) - v_m(owner trace), weighted.
+  if (visc) {
+    for (int k = tid; k < QF * 4; k += 64) {
+      const int q = k >> 2, m = k & 3;
+      const real w = a.fw[q] * area;
+      const real* phio = tbo + q * FO;
+      real vo = real(0), vn = real(0);
+#pragma unroll UO
+      for (int kk = 0; kk < FO; ++kk) vo += phio[kk] * svo[supo[kk] * 4 + m];
       if constexpr (NBR) {
         const real* phin = tbn + q * FN;
-#pragma unroll
-        for (int k = 0; k < FN; ++k) {
-          real b = phin[k];
-#pragma unroll
-          for (int m = 0; m < 4; ++m) vn[m] += b * svn[supn[k] * 4 + m];
-        }
+#pragma unroll UN
+        for (int kk = 0; kk < FN; ++kk) vn += phin[kk] * svn[supn[kk] * 4 + m];
       } else {
-        ghost_viscous<V, real>(sPo + q * V, n, flags, ph, vn);
+        real g[4];
+        ghost_viscous<V, real>(sPo + q * V, n, flags, ph, g);
+        vn = g[m];
       }
-#pragma unroll
-      for (int m = 0; m < 4; ++m) sJ[q * 4 + m] = w * (vn[m] - vo[m]);
+      sJ[k] = w * (vn - vo);
     }
   }
   __syncthreads();
@@ -989,26 +965,43 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     }
     __syncthreads();
 
-    // Phase 4: central viscous flux (owner flux only on the boundary).
-    for (int q = tid; q < QF; q += 64) {
-      const real w = a.fw[q] * area;
-      real Fo[3 * V], fn[V];
-      viscous_flux<V, real>(sPo + q * V, sgo + q * 12, gamma, ph.mu, ph.kappa, Fo);
-      if constexpr (NBR) {
-        real Fn[3 * V];
-        viscous_flux<V, real>(sPn + q * V, sgn + q * 12, gamma, ph.mu, ph.kappa, Fn);
+    // Phase 4: central viscous flux (owner flux only on the boundary), one
+    // lane per (point, side): lane QF + q evaluates the neighbour's flux at
+    // point q and hands its rows to lane q.  The viscous flux has momentum
+    // and energy rows only (the mass and species rows are zero): the other
+    // columns are left as they are.
+    static_assert(2 * QF <= 64, "two face sides per point in one wave");
+    {
+      const int side = (NBR && tid >= QF) ? 1 : 0;
+      const int q = tid - side * QF;
+      real Fx[3 * V];
 #pragma unroll
-        for (int k = 0; k < 3 * V; ++k) Fo[k] = real(0.5) * (Fo[k] + Fn[k]);
-      }
-      // the viscous flux has momentum and energy rows only (the mass and
-      // species rows are zero): the other columns are left as they are
+      for (int k = 0; k < 3 * V; ++k) Fx[k] = real(0);
+      if (q < QF)
+        viscous_flux<V, real>((side ? sPn : sPo) + q * V, (side ? sgn : sgo) + q * 12, gamma,
+                              ph.mu, ph.kappa, Fx);
+      real Fo[3][5];
 #pragma unroll
-      for (int v = 1; v < 5; ++v) fn[v] = Fo[v] * n[0] + Fo[V + v] * n[1] + Fo[2 * V + v] * n[2];
-      if (!NBR && ph.has_boundary && (flags & BND_WALL)) fn[4] = real(0);
+      for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int v = 1; v < 5; ++v) {
-        sSo[q * V + v] += w * fn[v];
-        sSn[q * V + v] -= w * fn[v];
+        for (int v = 1; v < 5; ++v) {
+          Fo[d][v] = Fx[d * V + v];
+          if constexpr (NBR) {
+            const real fnb = __shfl(Fx[d * V + v], tid + QF, 64);
+            Fo[d][v] = real(0.5) * (Fo[d][v] + fnb);
+          }
+        }
+      if (tid < QF) {
+        const real w = a.fw[q] * area;
+        real fn[5];
+#pragma unroll
+        for (int v = 1; v < 5; ++v) fn[v] = Fo[0][v] * n[0] + Fo[1][v] * n[1] + Fo[2][v] * n[2];
+        if (!NBR && ph.has_boundary && (flags & BND_WALL)) fn[4] = real(0);
+#pragma unroll
+        for (int v = 1; v < 5; ++v) {
+          sSo[q * V + v] += w * fn[v];
+          sSn[q * V + v] -= w * fn[v];
+        }
       }
     }
     __syncthreads();
