@@ -542,6 +542,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   __shared__ real sUU[NU], sAA[NA], svo[NO * 4], svn[NN * 4];
   __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
   __shared__ real sbo[FO * 12], sbn[FN * 12];
+  __shared__ real sB[2][6];              // anchor normal flux [5] and u.n per side
   real* const sUo = sUU;                 // [SO][V], slot 0 = node 0, slot 1 + k = support node k
   real* const sUn = sUU + SO * V;        // [SN][V]
   real* const sgo = sUU;                 // [QF][12] after phase 1b
@@ -607,6 +608,18 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
       }
   }
   __syncthreads();
+
+  // The anchor's normal flux and normal velocity, a constant of each side,
+  // once per face (owner on lane 0, neighbour on lane 1) for the Euler flux.
+  if (euler && tid < (NBR ? 2 : 1)) {
+    const real* u0 = tid ? sUn : sUo;
+    Prim<5, real> wa = c2p<5, real>(u0, gamma);
+    real B[5];
+    normal_flux<5, real>(u0, wa, n, B);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sB[tid][v] = B[v];
+    sB[tid][5] = wa.vel[0] * n[0] + wa.vel[1] * n[1] + wa.vel[2] * n[2];
+  }
 
   // Phase 1a: traces at the face points, one lane per (point, variable):
   // primal Up = sum_k phi_k u_k (for the viscous flux) and anchored
@@ -679,18 +692,18 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
         real F[5], Bo[5];
         FluxScalars<real> fs;
         numflux_split<V, real>(UL, UR, n, gamma, ph.rusanov, sl, sr, F, fs);
-        Prim<5, real> wa = c2p<5, real>(sUo, gamma);
-        normal_flux<5, real>(sUo, wa, n, Bo);
-        const real una = wa.vel[0] * n[0] + wa.vel[1] * n[1] + wa.vel[2] * n[2];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Bo[v] = sB[0][v];
+        const real una = sB[0][5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) sSo[q * V + v] = -w * (F[v] - Bo[v]);
         for (int sp = 5; sp < V; ++sp)
           sSo[q * V + sp] = -w * (species_flux(fs, sl(sp), sr(sp)) - sUo[sp] * una);
         if constexpr (NBR) {
           real Bn[5];
-          Prim<5, real> wb = c2p<5, real>(sUn, gamma);
-          normal_flux<5, real>(sUn, wb, n, Bn);
-          const real unb = wb.vel[0] * n[0] + wb.vel[1] * n[1] + wb.vel[2] * n[2];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Bn[v] = sB[1][v];
+          const real unb = sB[1][5];
 #pragma unroll
           for (int v = 0; v < 5; ++v) sSn[q * V + v] = w * (F[v] - Bn[v]);
           for (int sp = 5; sp < V; ++sp)
@@ -727,14 +740,15 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
       real Fs[V], Bo[V];
       if (ph.rusanov) rusanov<V, real>(Ua_o, Ua_n, n, gamma, Fs);
       else hllc<V, real>(Ua_o, Ua_n, n, gamma, Fs);
-      Prim<V, real> wa = c2p<V, real>(sUo, gamma);
-      normal_flux<V, real>(sUo, wa, n, Bo);
+      static_assert(V == 5, "flow variables only in this branch");
+#pragma unroll
+      for (int v = 0; v < V; ++v) Bo[v] = sB[0][v];
 #pragma unroll
       for (int v = 0; v < V; ++v) So[v] = -w * (Fs[v] - Bo[v]);
       if constexpr (NBR) {
         real Bn[V];
-        Prim<V, real> wb = c2p<V, real>(sUn, gamma);
-        normal_flux<V, real>(sUn, wb, n, Bn);
+#pragma unroll
+        for (int v = 0; v < V; ++v) Bn[v] = sB[1][v];
 #pragma unroll
         for (int v = 0; v < V; ++v) Sn[v] = w * (Fs[v] - Bn[v]);
       }

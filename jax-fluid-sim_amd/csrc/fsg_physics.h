@@ -203,18 +203,28 @@ __device__ __forceinline__ void hllc(const real* UL, const real* UR, const real*
   real SR = rmax(unl + cl, unr + cr);
   real den = wl.rho * (SL - unl) - wr.rho * (SR - unr);
   real SM = (wr.p - wl.p + wl.rho * unl * (SL - unl) - wr.rho * unr * (SR - unr)) / safe_signed(den);
-  real H[V];
-  if (SL >= real(0)) {
+  // One star-state evaluation on the side that SM selects (lanes of a wave
+  // that would take different sides then share one evaluation), then
+  // H = F_L (SL >= 0), star state (SM >= 0 or SR > 0), or F_R.
+  const bool lft = SM >= real(0);
+  real Ux[V], Fx[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) H[v] = FL[v];
-  } else if (SM >= real(0)) {
-    hllc_side<V, real>(UL, wl, n, SL, SM, FL, H);
-  } else if (SR > real(0)) {
-    hllc_side<V, real>(UR, wr, n, SR, SM, FR, H);
-  } else {
-#pragma unroll
-    for (int v = 0; v < V; ++v) H[v] = FR[v];
+  for (int v = 0; v < V; ++v) {
+    Ux[v] = lft ? UL[v] : UR[v];
+    Fx[v] = lft ? FL[v] : FR[v];
   }
+  Prim<V, real> wx;
+  wx.rho = lft ? wl.rho : wr.rho;
+  wx.safe = lft ? wl.safe : wr.safe;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) wx.vel[d] = lft ? wl.vel[d] : wr.vel[d];
+  wx.p = lft ? wl.p : wr.p;
+  real Hs[V];
+  hllc_side<V, real>(Ux, wx, n, lft ? SL : SR, SM, Fx, Hs);
+  const bool star = lft || SR > real(0);
+  real H[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) H[v] = SL >= real(0) ? FL[v] : (star ? Hs[v] : FR[v]);
   bool ok = (wl.rho > real(0)) && (wr.rho > real(0)) && (wl.p > real(0)) && (wr.p > real(0));
 #pragma unroll
   for (int v = 0; v < V; ++v) ok = ok && finite(H[v]);
