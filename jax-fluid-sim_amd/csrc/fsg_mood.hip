@@ -152,6 +152,28 @@ __device__ __forceinline__ real wmax(real x) {
   return x;
 }
 __device__ __forceinline__ bool wany(bool b) { return __ballot(b) != 0ull; }
+// Reductions over groups of L lanes (L = 64: the whole wave; L = 32: two
+// elements per wave).  With the inactive lanes of a group holding 0, the
+// L = 32 reduction gives the same value as the 64-lane one (its first step
+// would add or take the maximum with exact zeros).
+template <int L, typename real>
+__device__ __forceinline__ real gsum(real x) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+template <int L, typename real>
+__device__ __forceinline__ real gmax_num(real x) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+template <int L>
+__device__ __forceinline__ bool gany(bool b, int group) {
+  const unsigned long long m = __ballot(b);
+  if constexpr (L == 64) return m != 0ull;
+  else return ((m >> (group * L)) & ((1ull << L) - 1)) != 0ull;
+}
 // Wave maximum that ignores NaN (fmax): one instruction per step.  Used for
 // the deviation bounds of the sampling shortcuts, whose decision is false
 // anyway when an input is NaN or infinite (the mean, and with it the bounds,
@@ -322,13 +344,18 @@ __device__ void dg_point_flux(const Args<real>& a, const Phys<real>& ph, const G
 // Detection (hybrid/limiter.py:150-185, 285-300).  stage != 0: the detected
 // state is alpha base + beta (curr + dt (inv + visc)), and curr + dt (inv +
 // visc) is written to work; otherwise the detected state is det_src.
-template <int V, typename real>
+template <int V, typename real, int EPW>
 __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph, int64_t total) {
-  __shared__ real sU[4][MAXN][V];
-  __shared__ real sm[4][MAXN][2];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t g = (int64_t)blockIdx.x * 4 + w;
-  if (g >= total) return;   // whole wave exits together; no block barriers below
+  // EPW elements per wave, LG = 64 / EPW lanes each (EPW = 2 needs N <= 32)
+  constexpr int LG = 64 / EPW, NE = MAXN / EPW;
+  __shared__ real sU[4 * EPW][NE][V];
+  __shared__ real sm[4 * EPW][NE][2];
+  const int wl = threadIdx.x & 63, hh = wl / LG, lane = wl - hh * LG;
+  const int w = (threadIdx.x >> 6) * EPW + hh;
+  const int64_t g0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * EPW;
+  if (g0 >= total) return;  // whole wave exits together; no block barriers below
+  const bool live = g0 + hh < total;
+  const int64_t g = live ? g0 + hh : total - 1;   // a spare group repeats the last element
   int64_t e;
   const int b = locate(a, g, &e);
   const BlockT<real>& B = a.blocks[b];
@@ -345,16 +372,16 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
       const real* cur = a.curr[b] + o;
       const real* bas = a.base[b] + o;
       real* wrk = a.work[b] + o;
-      for (int k = lane; k < N * V; k += 64) {
+      for (int k = lane; k < N * V; k += LG) {
         real r = inv[k];
         if (vis) r = r + vis[k];
         const real eu = cur[k] + h * r;
-        wrk[k] = eu;
+        if (live) wrk[k] = eu;
         su[k] = a.alpha * bas[k] + a.beta * eu;
       }
     } else {
       const real* src = a.det_src[b] + o;
-      for (int k = lane; k < N * V; k += 64) su[k] = src[k];
+      for (int k = lane; k < N * V; k += LG) su[k] = src[k];
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -387,10 +414,10 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
       sc0 = m0 * m0;
       sc1 = m1 * m1;
     }
-    hi0 = wsum(hi0);
-    hi1 = wsum(hi1);
-    sc0 = wsum(sc0);
-    sc1 = wsum(sc1);
+    hi0 = gsum<LG>(hi0);
+    hi1 = gsum<LG>(hi1);
+    sc0 = gsum<LG>(sc0);
+    sc1 = gsum<LG>(sc1);
     const real r0 = hi0 / (sc0 > real(1e-12) ? sc0 : real(1e-12));
     const real r1 = hi1 / (sc1 > real(1e-12) ? sc1 : real(1e-12));
     ratio = nmax(r0, r1);
@@ -409,14 +436,14 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
       for (int j = 0; j < N; ++j) mpart += T.mw[j] * sU[w][j][lane];
     real mv[5], D[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) mv[v] = __shfl(mpart, v, 64) / T.wsum;
+    for (int v = 0; v < 5; ++v) mv[v] = __shfl(mpart, hh * LG + v, 64) / T.wsum;
     const real big = sizeof(real) == 4 ? real(1e15) : real(1e150);
     bool fine = true;
     if (lane < N)
 #pragma unroll
       for (int v = 0; v < V; ++v) fine = fine && isnum(sU[w][lane][v]) && fabs(sU[w][lane][v]) <= big;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) D[v] = wmax_num(lane < N ? fabs(sU[w][lane][v] - mv[v]) : real(0));
+    for (int v = 0; v < 5; ++v) D[v] = gmax_num<LG>(lane < N ? fabs(sU[w][lane][v] - mv[v]) : real(0));
     const real L = T.lebesgue;
     const real eps = sizeof(real) == 4 ? real(1.2e-7) : real(2.3e-16);
     const real rnd = real(16) * real(N + 2) * eps * L;
@@ -433,7 +460,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     const real ke_ub = rho_lb > real(0) ? real(0.5) * m2 / rho_lb : real(0);
     const real p_lb = (ph.gamma - real(1)) * (mv[4] - bnd[4] - ke_ub);
     const real p_tol = real(16) * eps * (ph.gamma - real(1)) * (fabs(mv[4]) + bnd[4] + ke_ub);
-    pad_done = !wany(!fine) && rho_lb > real(0) && rho_lb >= a.dfloor &&
+    pad_done = !gany<LG>(!fine, hh) && rho_lb > real(0) && rho_lb >= a.dfloor &&
                p_lb - p_tol >= a.pfloor;
   }
   // PAD over nodes, volume points, face sample points and the mean.
@@ -451,7 +478,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
   };
   if (!pad_done) {
   if (lane < N) bad = bad || check(sU[w][lane]);
-  for (int k0 = 0; k0 < T.NS; k0 += 64) {
+  for (int k0 = 0; k0 < T.NS; k0 += LG) {
     const int k = k0 + lane;
     if (k < T.NS) {
       real u[V];
@@ -471,11 +498,11 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
   }
   real mean[V];
 #pragma unroll
-  for (int v = 0; v < V; ++v) mean[v] = wsum(msum[v]) / T.wsum;
+  for (int v = 0; v < V; ++v) mean[v] = gsum<LG>(msum[v]) / T.wsum;
   bad = bad || check(mean);
   }
-  const bool troubled = wany(bad) || ratio > a.thr;
-  if (lane == 0) {
+  const bool troubled = gany<LG>(bad, hh) || ratio > a.thr;
+  if (lane == 0 && live) {
     B.flags[e] = troubled ? 1 : 0;
     if (B.sensor) B.sensor[e] = ratio;
   }
@@ -1018,6 +1045,7 @@ struct fsg_mood {
   void* d_groups = nullptr;
   std::vector<void*> d_mw;      // per-type node weights of the volume mean
   std::vector<int> block_ks;    // samples per lane of the positivity kernel, per block
+  int max_n = 0;                // largest node count of the element types in use
   std::vector<int64_t> offsets;
   std::vector<int32_t*> flags;
 };
@@ -1102,6 +1130,7 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
     d.mw = static_cast<const real*>(dmw);
     d.KS = (s.N + s.NS + 63) / 64;
     type_ks[t] = d.KS;
+    if (s.N > m->max_n) m->max_n = s.N;
   }
   const int nb = (int)c->blocks.size();
   std::vector<BlockT<real>> bb(nb);
@@ -1231,8 +1260,14 @@ constexpr unsigned SUBCELL_GRID = 8192;
 // detect (+ compaction)
 template <int V, typename real>
 int run_detect(fsg_mood* m, Args<real>& a, const Phys<real>& ph, hipStream_t s) {
-  const unsigned g4 = (unsigned)((m->total + 3) / 4), g256 = (unsigned)((m->total + 255) / 256);
-  hipLaunchKernelGGL((detect_kernel<V, real>), dim3(g4), dim3(256), 0, s, a, ph, m->total);
+  const unsigned g256 = (unsigned)((m->total + 255) / 256);
+  if (m->max_n <= MAXN / 2) {   // two elements per wave (p <= 2)
+    const unsigned g8 = (unsigned)((m->total + 7) / 8);
+    hipLaunchKernelGGL((detect_kernel<V, real, 2>), dim3(g8), dim3(256), 0, s, a, ph, m->total);
+  } else {
+    const unsigned g4 = (unsigned)((m->total + 3) / 4);
+    hipLaunchKernelGGL((detect_kernel<V, real, 1>), dim3(g4), dim3(256), 0, s, a, ph, m->total);
+  }
   if (int rc = launched("MOOD detect")) return rc;
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, m->counter);
   hipLaunchKernelGGL((compact_kernel<real>), dim3(g256), dim3(256), 0, s, a, m->total);
