@@ -74,6 +74,8 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t e0 = (int64_t)blockIdx.x * EB;
+  DiagSeen seen{};
+  if (a.mode == 2 && lane == 0) seen = diag_seen(a.diag);
   const int ne = (int)(count - e0 < EB ? count - e0 : EB);
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
@@ -430,8 +432,8 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
     bad += __shfl_down(bad, off, 64);
   }
   if (lane == 0) {
-    atomic_min_nan<real>(&a.diag->rho_min, rmn);
-    atomic_min_nan<real>(&a.diag->p_min, pmn);
+    atomic_min_nan<real>(&a.diag->rho_min, rmn, seen.rho);
+    atomic_min_nan<real>(&a.diag->p_min, pmn, seen.p);
     if (bad) atomicAdd(&a.diag->nonfinite, bad);
   }
 }

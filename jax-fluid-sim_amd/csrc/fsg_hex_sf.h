@@ -115,15 +115,31 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   const int tid = threadIdx.x;
   const int64_t e = blockIdx.x;
   if (a.mask && a.mask[e] == 0) return;   // whole workgroup: no barrier passed yet
+  DiagSeen seen{};
+  if (a.mode == 2 && tid == 0) seen = diag_seen(a.diag);
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;
 
   const real* Ue = a.U + e * (N * V);
-  for (int k = tid; k < N * V; k += NT) {
-    sU[k] = Ue[k];
-    if (k < V) sA[k] = Ue[k];
-    sR[k] = real(0);
+  {
+    // every load of the element state in flight before the first is used
+    constexpr int IU = (N * V + NT - 1) / NT;
+    real t[IU];
+#pragma unroll
+    for (int i = 0; i < IU; ++i) {
+      const int k = tid + NT * i;
+      t[i] = Ue[k < N * V ? k : N * V - 1];
+    }
+#pragma unroll
+    for (int i = 0; i < IU; ++i) {
+      const int k = tid + NT * i;
+      if (k < N * V) {
+        sU[k] = t[i];
+        if (k < V) sA[k] = t[i];
+        sR[k] = real(0);
+      }
+    }
   }
   for (int k = tid; k < N * 12; k += NT) sb[k] = real(0);
   for (int k = tid; k < hex_sf_table_size<P>(); k += NT) {
@@ -138,23 +154,7 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
       viscous_variables<V, real>(sU + i * V, gamma, ph.gas_constant, sv + i * 4);
 
   // Face-side records: surface terms into sR, lift right-hand sides into sb.
-  const int64_t r0 = a.rec_ptr[e], r1 = a.rec_ptr[e + 1];
-  for (int64_t r = r0; r < r1; ++r) {
-    const real* rec = a.face_out + a.rec_off[r];
-    const int32_t* sup = a.support + a.rec_sup[r];
-    const int nf = a.rec_nf[r];
-    const int width = visc ? W : V;
-    for (int k = tid; k < nf * width; k += NT) {
-      int kk = k / width, c = k - kk * width;
-      int node = sup[kk];
-      real val = rec[kk * W + c];
-      if (c < V)
-        sR[node * V + c] += val;
-      else
-        sb[node * 12 + (c - V)] += val;
-    }
-    __syncthreads();
-  }
+  add_face_records<NT, (N1 * N1 * W + NT - 1) / NT, V, real>(a, e, visc, sR, sb, tid);
   __syncthreads();
 
   const real s = a.scale[e];
@@ -429,8 +429,8 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
       pmn = nan_min(pmn, red[1][k]);
       bad += redn[k];
     }
-    atomic_min_nan<real>(&a.diag->rho_min, rmn);
-    atomic_min_nan<real>(&a.diag->p_min, pmn);
+    atomic_min_nan<real>(&a.diag->rho_min, rmn, seen.rho);
+    atomic_min_nan<real>(&a.diag->p_min, pmn, seen.p);
     if (bad) atomicAdd(&a.diag->nonfinite, bad);
   }
 }

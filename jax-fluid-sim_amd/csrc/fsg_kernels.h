@@ -88,13 +88,31 @@ struct FaceArgs {
   const int32_t* mask_n;   // with a marked side, or null: all faces
 };
 
+// The two minima of the stage diagnostics as they stand, read coherently
+// (agent scope: a plain load may see a stale cached line).  A stored minimum
+// only moves down while a launch runs (NaN sits below every number), so a
+// value that does not beat an earlier read cannot beat the current minimum:
+// kernels read the minima when they start, where the read's latency hides
+// behind their work, and hand them to atomic_min_nan at the end.
+struct DiagSeen {
+  double rho, p;
+};
+__device__ __forceinline__ DiagSeen diag_seen(StageDiag* d) {
+  DiagSeen s;
+  s.rho = __longlong_as_double(static_cast<long long>(__hip_atomic_load(
+      reinterpret_cast<unsigned long long*>(&d->rho_min), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  s.p = __longlong_as_double(static_cast<long long>(__hip_atomic_load(
+      reinterpret_cast<unsigned long long*>(&d->p_min), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  return s;
+}
+
+// NaN-propagating atomic minimum, starting from `seen`, an earlier coherent
+// read of *addr: almost every caller finds the minimum already below its
+// value and returns without a read-modify-write.
 template <typename real>
-__device__ __forceinline__ void atomic_min_nan(double* addr, double v) {
+__device__ __forceinline__ void atomic_min_nan(double* addr, double v, double seen) {
   unsigned long long* p = reinterpret_cast<unsigned long long*>(addr);
-  // coherent read first: almost every caller finds the minimum already below
-  // its value and returns without a read-modify-write (a plain load may see a
-  // stale cached line and send every workgroup into the CAS loop)
-  unsigned long long old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long old = static_cast<unsigned long long>(__double_as_longlong(seen));
   unsigned long long assumed;
   do {
     assumed = old;
@@ -118,6 +136,95 @@ __host__ __device__ constexpr int div_split(int N, int NT) {
   return d;
 }
 
+__device__ __forceinline__ int64_t readlane64(int64_t x, int j) {
+  const unsigned lo = __builtin_amdgcn_readlane((int)(uint32_t)x, j);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Face-side records of element e: surface terms into sR [N][V] and, with the
+// viscous terms, lift right-hand sides into sb [N][12]; record by record, a
+// barrier after each (records of one element share edge and corner nodes),
+// so the sums keep the record order.  The element's record descriptors are
+// read 64 at a time (lane j of every wave holds record j, broadcast with
+// readlane) and the values of the next record are loaded while the current
+// one is added: a record costs about one memory round trip instead of three
+// dependent ones.  A thread holds RI items of a record, enough for a face of
+// the element; items past RI NT (none for such faces) are read directly.
+template <int NT, int RI, int V, typename real>
+__device__ __forceinline__ void add_face_records(const ElemArgs<real>& a, int64_t e, bool visc,
+                                                 real* sR, real* sb, int tid) {
+  constexpr int W = V + 12;
+  const int width = visc ? W : V;
+  const int64_t r0 = a.rec_ptr[e], r1 = a.rec_ptr[e + 1];
+  const int lt = tid & 63;
+  for (int64_t rb = r0; rb < r1; rb += 64) {
+    const int nr = (int)(r1 - rb < 64 ? r1 - rb : 64);
+    int64_t moff = 0;
+    int msup = 0, mnf = 0;
+    if (lt < nr) {
+      moff = a.rec_off[rb + lt];
+      msup = a.rec_sup[rb + lt];
+      mnf = a.rec_nf[rb + lt];
+    }
+    auto fetch = [&](int j, real (&v)[RI], int (&nd)[RI], int (&cl)[RI]) {
+      const real* rec = a.face_out + readlane64(moff, j);
+      const int32_t* sup = a.support + __builtin_amdgcn_readlane(msup, j);
+      const int cnt = __builtin_amdgcn_readlane(mnf, j) * width;
+#pragma unroll
+      for (int i = 0; i < RI; ++i) {
+        const int k = tid + NT * i;
+        nd[i] = -1;
+        cl[i] = 0;
+        v[i] = real(0);
+        if (k < cnt) {
+          const int kk = k / width, c = k - kk * width;
+          nd[i] = sup[kk];
+          cl[i] = c;
+          v[i] = rec[kk * W + c];
+        }
+      }
+    };
+    auto add = [&](int j, const real (&v)[RI], const int (&nd)[RI], const int (&cl)[RI]) {
+#pragma unroll
+      for (int i = 0; i < RI; ++i) {
+        if (nd[i] < 0) continue;
+        if (cl[i] < V)
+          sR[nd[i] * V + cl[i]] += v[i];
+        else
+          sb[nd[i] * 12 + (cl[i] - V)] += v[i];
+      }
+      const int cnt = __builtin_amdgcn_readlane(mnf, j) * width;
+      if (cnt > RI * NT) {
+        const real* rec = a.face_out + readlane64(moff, j);
+        const int32_t* sup = a.support + __builtin_amdgcn_readlane(msup, j);
+        for (int k = RI * NT + tid; k < cnt; k += NT) {
+          const int kk = k / width, c = k - kk * width;
+          const int node = sup[kk];
+          const real val = rec[kk * W + c];
+          if (c < V)
+            sR[node * V + c] += val;
+          else
+            sb[node * 12 + (c - V)] += val;
+        }
+      }
+    };
+    real va[RI], vb[RI];
+    int na[RI], nb[RI], ca[RI], cb[RI];
+    fetch(0, va, na, ca);
+    for (int j = 0; j < nr; j += 2) {
+      if (j + 1 < nr) fetch(j + 1, vb, nb, cb);
+      add(j, va, na, ca);
+      __syncthreads();
+      if (j + 1 < nr) {
+        if (j + 2 < nr) fetch(j + 2, va, na, ca);
+        add(j + 1, vb, nb, cb);
+        __syncthreads();
+      }
+    }
+  }
+}
+
 template <int T, int P, int V, typename real, int NT>
 __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real> ph) {
   constexpr int N = n_nodes(T, P);
@@ -138,14 +245,30 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   const int tid = threadIdx.x;
   const int64_t e = blockIdx.x;
   if (a.mask && a.mask[e] == 0) return;   // whole workgroup: no barrier passed yet
+  DiagSeen seen{};
+  if (a.mode == 2 && tid == 0) seen = diag_seen(a.diag);
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;
 
   const real* Ue = a.U + e * (N * V);
-  for (int k = tid; k < N * V; k += NT) {
-    sU[k] = Ue[k];
-    sR[k] = real(0);
+  {
+    // every load of the element state in flight before the first is used
+    constexpr int IU = (N * V + NT - 1) / NT;
+    real t[IU];
+#pragma unroll
+    for (int i = 0; i < IU; ++i) {
+      const int k = tid + NT * i;
+      t[i] = Ue[k < N * V ? k : N * V - 1];
+    }
+#pragma unroll
+    for (int i = 0; i < IU; ++i) {
+      const int k = tid + NT * i;
+      if (k < N * V) {
+        sU[k] = t[i];
+        sR[k] = real(0);
+      }
+    }
   }
   for (int k = tid; k < N * 12; k += NT) sb[k] = real(0);
   __syncthreads();
@@ -154,23 +277,7 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
       viscous_variables<V, real>(sU + i * V, gamma, ph.gas_constant, sv + i * 4);
 
   // Face-side records: surface terms into sR, lift right-hand sides into sb.
-  const int64_t r0 = a.rec_ptr[e], r1 = a.rec_ptr[e + 1];
-  for (int64_t r = r0; r < r1; ++r) {
-    const real* rec = a.face_out + a.rec_off[r];
-    const int32_t* sup = a.support + a.rec_sup[r];
-    const int nf = a.rec_nf[r];
-    const int width = visc ? W : V;
-    for (int k = tid; k < nf * width; k += NT) {
-      int kk = k / width, c = k - kk * width;
-      int node = sup[kk];
-      real val = rec[kk * W + c];
-      if (c < V)
-        sR[node * V + c] += val;
-      else
-        sb[node * 12 + (c - V)] += val;
-    }
-    __syncthreads();
-  }
+  add_face_records<NT, ((P + 1) * (P + 1) * W + NT - 1) / NT, V, real>(a, e, visc, sR, sb, tid);
 
   const real s = a.scale[e];
   const real dw = a.detw[e];
@@ -411,8 +518,8 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
       pmn = nan_min(pmn, red[1][k]);
       bad += redn[k];
     }
-    atomic_min_nan<real>(&a.diag->rho_min, rmn);
-    atomic_min_nan<real>(&a.diag->p_min, pmn);
+    atomic_min_nan<real>(&a.diag->rho_min, rmn, seen.rho);
+    atomic_min_nan<real>(&a.diag->p_min, pmn, seen.p);
     if (bad) atomicAdd(&a.diag->nonfinite, bad);
   }
 }

@@ -193,6 +193,28 @@ __device__ __forceinline__ int locate(const Args<real>& a, int64_t g, int64_t* e
   return b;
 }
 
+// dst[k] = src[k] for k < n, lane l of a group of S lanes taking k = l, l + S,
+// ...  The loads go out in batches of CH, all issued before the first is
+// waited on: a loop that stores each value before it loads the next waits
+// one memory round trip per S values.  Indices past n load src[n - 1] (n > 0)
+// and are not stored.
+template <int S, int CH, typename real>
+__device__ __forceinline__ void load_rows(real* dst, const real* src, int n, int lane) {
+  for (int k0 = 0; k0 < n; k0 += S * CH) {
+    real t[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int k = k0 + lane + S * i;
+      t[i] = src[k < n ? k : n - 1];
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int k = k0 + lane + S * i;
+      if (k < n) dst[k] = t[i];
+    }
+  }
+}
+
 // hybrid/solver.py:83-98 pressure of a conserved state.
 template <typename real>
 __device__ __forceinline__ real pressure(const real* U, real gamma) {
@@ -350,6 +372,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
   constexpr int LG = 64 / EPW, NE = MAXN / EPW;
   __shared__ real sU[4 * EPW][NE][V];
   __shared__ real sm[4 * EPW][NE][2];
+  __shared__ real sW[4 * EPW][NE];   // the type's basis-weight products mw
   const int wl = threadIdx.x & 63, hh = wl / LG, lane = wl - hh * LG;
   const int w = (threadIdx.x >> 6) * EPW + hh;
   const int64_t g0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * EPW;
@@ -365,6 +388,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     // element values in node-major order, one coalesced pass over N V
     real* su = &sU[w][0][0];
     const int64_t o = e * N * V;
+    const real mwl = lane < N ? T.mw[lane] : real(0);
     if (a.stage) {
       const real h = a.dte[b] ? a.dte[b][e] : a.dt;
       const real* inv = a.inv[b] + o;
@@ -372,17 +396,35 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
       const real* cur = a.curr[b] + o;
       const real* bas = a.base[b] + o;
       real* wrk = a.work[b] + o;
-      for (int k = lane; k < N * V; k += LG) {
-        real r = inv[k];
-        if (vis) r = r + vis[k];
-        const real eu = cur[k] + h * r;
-        if (live) wrk[k] = eu;
-        su[k] = a.alpha * bas[k] + a.beta * eu;
+      // batches of CH indices per lane, every load of a batch in flight at once
+      constexpr int CH = 4;
+      const int n = N * V;
+      for (int k0 = 0; k0 < n; k0 += LG * CH) {
+        real ri[CH], rv[CH], rc[CH], rb[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int k = k0 + lane + LG * i, kk = k < n ? k : n - 1;
+          ri[i] = inv[kk];
+          rc[i] = cur[kk];
+          rb[i] = bas[kk];
+          if (vis) rv[i] = vis[kk];
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int k = k0 + lane + LG * i;
+          if (k < n) {
+            real r = ri[i];
+            if (vis) r = r + rv[i];
+            const real eu = rc[i] + h * r;
+            if (live) wrk[k] = eu;
+            su[k] = a.alpha * rb[i] + a.beta * eu;
+          }
+        }
       }
     } else {
-      const real* src = a.det_src[b] + o;
-      for (int k = lane; k < N * V; k += LG) su[k] = src[k];
+      load_rows<LG, 8>(su, a.det_src[b] + o, N * V, lane);
     }
+    if (lane < N) sW[w][lane] = mwl;
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -433,7 +475,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     real mpart = real(0);
     if (lane < 5)
 #pragma unroll 8
-      for (int j = 0; j < N; ++j) mpart += T.mw[j] * sU[w][j][lane];
+      for (int j = 0; j < N; ++j) mpart += sW[w][j] * sU[w][j][lane];
     real mv[5], D[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) mv[v] = __shfl(mpart, hh * LG + v, 64) / T.wsum;
@@ -844,15 +886,15 @@ __global__ __launch_bounds__(256) void correct_kernel(Args<real> a, Phys<real> p
 // interpolant, with the basis-weight products summed once per type); lane l
 // holds the deviations of samples l, l + 64, ... (nodes, then the volume and
 // face sample points) in registers, KS of them.  Writes the limited state
-// back to sU.
+// back to sU.  sW holds the type's basis-weight products mw.
 template <int V, int KS, typename real>
 __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const TypeT<real>& T,
-                                real (*sU)[V], real* sM, int lane) {
+                                real (*sU)[V], real* sM, const real* sW, int lane) {
   const int N = T.N, NT = N + T.NS;
   if (lane < V) {
     real m = real(0);
 #pragma unroll 8
-    for (int j = 0; j < N; ++j) m += T.mw[j] * sU[j][lane];
+    for (int j = 0; j < N; ++j) m += sW[j] * sU[j][lane];
     sM[lane] = m / T.wsum;
   }
   __builtin_amdgcn_wave_barrier();
@@ -986,6 +1028,7 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
                                                         int64_t g0) {
   __shared__ real sU[MAXN][V];
   __shared__ real sM[V];
+  __shared__ real sW[MAXN];
   const int lane = threadIdx.x;
   int64_t e;
   const int b = locate(a, g0 + (int64_t)blockIdx.x, &e);
@@ -993,17 +1036,35 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
   const TypeT<real>& T = a.types[B.type];
   const int N = T.N;
   const real* src = mode == 0 ? a.det_src[b] : a.work[b];
-  for (int k = lane; k < N * V; k += 64) sU[k / V][k % V] = src[e * N * V + k];
+  fsg::DiagSeen seen{};
+  if (mode == 1 && a.diag && lane == 0) seen = fsg::diag_seen(a.diag);
+  const real mwl = lane < N ? T.mw[lane] : real(0);
+  load_rows<64, 8>(&sU[0][0], src + e * N * V, N * V, lane);
+  if (lane < N) sW[lane] = mwl;
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  positivity_wave<V, KS, real>(a, ph, T, sU, sM, lane);
+  positivity_wave<V, KS, real>(a, ph, T, sU, sM, sW, lane);
   if (mode == 1) {
     const real* bs = a.base[b] + e * N * V;
-    for (int k = lane; k < N * V; k += 64)
-      sU[k / V][k % V] = a.alpha * bs[k] + a.beta * sU[k / V][k % V];
+    real* su = &sU[0][0];
+    constexpr int CH = 8;
+    const int n = N * V;
+    for (int k0 = 0; k0 < n; k0 += 64 * CH) {
+      real t[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int k = k0 + lane + 64 * i;
+        t[i] = bs[k < n ? k : n - 1];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int k = k0 + lane + 64 * i;
+        if (k < n) su[k] = a.alpha * t[i] + a.beta * su[k];
+      }
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    positivity_wave<V, KS, real>(a, ph, T, sU, sM, lane);
+    positivity_wave<V, KS, real>(a, ph, T, sU, sM, sW, lane);
   }
   real* out = a.out[b] + e * N * V;
   for (int k = lane; k < N * V; k += 64) out[k] = sU[k / V][k % V];
@@ -1023,8 +1084,8 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
     pmin = wmin(pmin);
     const bool nf = wany(nonfinite);
     if (lane == 0) {
-      fsg::atomic_min_nan<real>(&a.diag->rho_min, (double)rmin);
-      fsg::atomic_min_nan<real>(&a.diag->p_min, (double)pmin);
+      fsg::atomic_min_nan<real>(&a.diag->rho_min, (double)rmin, seen.rho);
+      fsg::atomic_min_nan<real>(&a.diag->p_min, (double)pmin, seen.p);
       if (nf) atomicAdd(&a.diag->nonfinite, 1u);
     }
   }

@@ -24,7 +24,7 @@ regression can follow; that mode can still be selected.
 
 A cell whose features leave the training hull (plus a margin) or whose
 projection misses the tolerance is flagged and advanced by the plain
-integrator instead, which therefore also judges the surrogate's error.
+integrator instead, which therefore also measures the surrogate's error.
 """
 
 import ctypes
