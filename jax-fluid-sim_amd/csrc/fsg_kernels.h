@@ -684,35 +684,74 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   }
 
   // stage node 0 (the anchor) and the face-support nodes of each side; the
-  // viscous variables of every node come from the flow columns directly
+  // viscous variables of every node come from the flow columns directly.
+  // Every load of both sides is issued before the first value is used: the
+  // support indices, then the staged states and the viscous nodes' flow
+  // columns (a loop that stores each value before loading the next waits
+  // two dependent round trips per 64 values and side).
   {
+    constexpr int IO = (SO * V + 63) / 64, IN = (SN * V + 63) / 64;
+    constexpr int VO = (NO + 63) / 64, VN = NBR ? (NN + 63) / 64 : 1;
     const real* Ue = a.Uo + eo * (NO * V);
-    for (int k = tid; k < SO * V; k += 64) {
-      const int sl = k / V, v = k - sl * V;
-      sUo[k] = Ue[(sl == 0 ? 0 : supo[sl - 1]) * V + v];
+    const real* Un = a.Un + en * (NN * V);
+    int ro[IO], rn[IN];
+#pragma unroll
+    for (int i = 0; i < IO; ++i) {
+      const int k = tid + 64 * i < SO * V ? tid + 64 * i : SO * V - 1, sl = k / V;
+      const int s = supo[sl > 0 ? sl - 1 : 0];
+      ro[i] = (sl == 0 ? 0 : s) * V + (k - sl * V);
     }
     if constexpr (NBR) {
-      const real* Un = a.Un + en * (NN * V);
-      for (int k = tid; k < SN * V; k += 64) {
-        const int sl = k / V, v = k - sl * V;
-        sUn[k] = Un[(sl == 0 ? 0 : supn[sl - 1]) * V + v];
+#pragma unroll
+      for (int i = 0; i < IN; ++i) {
+        const int k = tid + 64 * i < SN * V ? tid + 64 * i : SN * V - 1, sl = k / V;
+        const int s = supn[sl > 0 ? sl - 1 : 0];
+        rn[i] = (sl == 0 ? 0 : s) * V + (k - sl * V);
       }
     }
-  }
-  if (visc) {
-    for (int i = tid; i < NO; i += 64) {
-      real u[5];
+    real fo[VO][5], fn[VN][5];
+    if (visc) {
 #pragma unroll
-      for (int v = 0; v < 5; ++v) u[v] = a.Uo[eo * (NO * V) + i * V + v];
-      viscous_variables<5, real>(u, gamma, ph.gas_constant, svo + i * 4);
-    }
-    if constexpr (NBR)
-      for (int i = tid; i < NN; i += 64) {
-        real u[5];
+      for (int j = 0; j < VO; ++j) {
+        const int i = tid + 64 * j < NO ? tid + 64 * j : NO - 1;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) u[v] = a.Un[en * (NN * V) + i * V + v];
-        viscous_variables<5, real>(u, gamma, ph.gas_constant, svn + i * 4);
+        for (int v = 0; v < 5; ++v) fo[j][v] = Ue[i * V + v];
       }
+      if constexpr (NBR) {
+#pragma unroll
+        for (int j = 0; j < VN; ++j) {
+          const int i = tid + 64 * j < NN ? tid + 64 * j : NN - 1;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) fn[j][v] = Un[i * V + v];
+        }
+      }
+    }
+    real uo[IO], un[IN];
+#pragma unroll
+    for (int i = 0; i < IO; ++i) uo[i] = Ue[ro[i]];
+    if constexpr (NBR) {
+#pragma unroll
+      for (int i = 0; i < IN; ++i) un[i] = Un[rn[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < IO; ++i)
+      if (tid + 64 * i < SO * V) sUo[tid + 64 * i] = uo[i];
+    if constexpr (NBR) {
+#pragma unroll
+      for (int i = 0; i < IN; ++i)
+        if (tid + 64 * i < SN * V) sUn[tid + 64 * i] = un[i];
+    }
+    if (visc) {
+#pragma unroll
+      for (int j = 0; j < VO; ++j)
+        if (tid + 64 * j < NO) viscous_variables<5, real>(fo[j], gamma, ph.gas_constant, svo + (tid + 64 * j) * 4);
+      if constexpr (NBR) {
+#pragma unroll
+        for (int j = 0; j < VN; ++j)
+          if (tid + 64 * j < NN)
+            viscous_variables<5, real>(fn[j], gamma, ph.gas_constant, svn + (tid + 64 * j) * 4);
+      }
+    }
   }
   __syncthreads();
 

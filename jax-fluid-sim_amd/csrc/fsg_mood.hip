@@ -1336,8 +1336,8 @@ int run_detect(fsg_mood* m, Args<real>& a, const Phys<real>& ph, hipStream_t s) 
 }
 
 // One launch per element block, each with the sample count per lane of its
-// element type (hexahedra need 2 at p = 2 where tetrahedra need 8: the
-// deviation registers, and with them the occupancy, follow the block).
+// element type (hexahedra and wedges need 5 at p = 2 where tetrahedra need
+// 8: the deviation registers, and with them the occupancy, follow the block).
 template <int V, typename real>
 void launch_positivity(fsg_mood* m, const Args<real>& a, const Phys<real>& ph, int mode,
                        hipStream_t s) {
