@@ -109,6 +109,8 @@ struct Args {
   const BlockT<real>* blocks;
   const GroupT<real>* groups;
   int nblocks;
+  int64_t boff[MAXB + 1];   // global index of each block's first element, then the total
+  int btype[MAXB];          // element type of each block
   const real* curr[MAXB];   // states the subcell update starts from
   const real* base[MAXB];   // stage base
   const real* inv[MAXB];    // inviscid rhs (stage)
@@ -188,8 +190,8 @@ __device__ __forceinline__ real wmax_num(real x) {
 template <typename real>
 __device__ __forceinline__ int locate(const Args<real>& a, int64_t g, int64_t* e) {
   int b = 0;
-  while (b + 1 < a.nblocks && g >= a.blocks[b + 1].offset) ++b;
-  *e = g - a.blocks[b].offset;
+  while (b + 1 < a.nblocks && g >= a.boff[b + 1]) ++b;
+  *e = g - a.boff[b];
   return b;
 }
 
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
   int64_t e;
   const int b = locate(a, g, &e);
   const BlockT<real>& B = a.blocks[b];
-  const TypeT<real>& T = a.types[B.type];
+  const TypeT<real>& T = a.types[a.btype[b]];
   const int N = T.N;
   {
     // element values in node-major order, one coalesced pass over N V
@@ -591,7 +593,7 @@ __global__ __launch_bounds__(64) void subcell_kernel(Args<real> a, Phys<real> ph
     int64_t e;
     const int b = locate(a, (int64_t)a.list[item], &e);
     const BlockT<real>& B = a.blocks[b];
-    const TypeT<real>& T = a.types[B.type];
+    const TypeT<real>& T = a.types[a.btype[b]];
     const int N = T.N;
     const real* U = a.curr[b] + e * N * V;
     for (int k = s; k < N * V; k += 64) sU[k / V][k % V] = U[k];
@@ -845,7 +847,7 @@ __global__ __launch_bounds__(256) void correct_kernel(Args<real> a, Phys<real> p
     int64_t e;
     const int b = locate(a, g0 + l, &e);
     const BlockT<real>& B = a.blocks[b];
-    const TypeT<real>& T = a.types[B.type];
+    const TypeT<real>& T = a.types[a.btype[b]];
     real corr[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) corr[v] = real(0);
@@ -1033,7 +1035,7 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
   int64_t e;
   const int b = locate(a, g0 + (int64_t)blockIdx.x, &e);
   const BlockT<real>& B = a.blocks[b];
-  const TypeT<real>& T = a.types[B.type];
+  const TypeT<real>& T = a.types[a.btype[b]];
   const int N = T.N;
   const real* src = mode == 0 ? a.det_src[b] : a.work[b];
   fsg::DiagSeen seen{};
@@ -1105,7 +1107,8 @@ struct fsg_mood {
   void* d_blocks = nullptr;
   void* d_groups = nullptr;
   std::vector<void*> d_mw;      // per-type node weights of the volume mean
-  std::vector<int> block_ks;    // samples per lane of the positivity kernel, per block
+  std::vector<int> block_ks;
+  std::vector<int> block_type;  // element type per block    // samples per lane of the positivity kernel, per block
   int max_n = 0;                // largest node count of the element types in use
   std::vector<int64_t> offsets;
   std::vector<int32_t*> flags;
@@ -1210,6 +1213,7 @@ int upload(fsg_mood* m, const fsg_mood_type* types, const fsg_mood_block* blocks
     d.sensor = static_cast<real*>(blocks[b].sensor);
     m->offsets.push_back(off);
     m->block_ks.push_back(type_ks[cb.element_type]);
+    m->block_type.push_back(cb.element_type);
     m->flags.push_back(blocks[b].flags);
     off += cb.count;
   }
@@ -1275,6 +1279,11 @@ Args<real> base_args(fsg_mood* m) {
   a.blocks = static_cast<const BlockT<real>*>(m->d_blocks);
   a.groups = static_cast<const GroupT<real>*>(m->d_groups);
   a.nblocks = (int)m->ctx->blocks.size();
+  for (int b = 0; b < a.nblocks; ++b) {
+    a.boff[b] = m->offsets[b];
+    a.btype[b] = m->block_type[b];
+  }
+  a.boff[a.nblocks] = m->total;
   a.counter = m->counter;
   a.list = m->list;
   a.thr = real(m->params.threshold);
