@@ -1038,8 +1038,6 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
   const TypeT<real>& T = a.types[a.btype[b]];
   const int N = T.N;
   const real* src = mode == 0 ? a.det_src[b] : a.work[b];
-  fsg::DiagSeen seen{};
-  if (mode == 1 && a.diag && lane == 0) seen = fsg::diag_seen(a.diag);
   const real mwl = lane < N ? T.mw[lane] : real(0);
   load_rows<64, 8>(&sU[0][0], src + e * N * V, N * V, lane);
   if (lane < N) sW[lane] = mwl;
@@ -1086,6 +1084,9 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
     pmin = wmin(pmin);
     const bool nf = wany(nonfinite);
     if (lane == 0) {
+      // read here, not when the wave starts: with one wave per element, half a
+      // million waves reading one address up front cost more than they hid
+      const fsg::DiagSeen seen = fsg::diag_seen(a.diag);
       fsg::atomic_min_nan<real>(&a.diag->rho_min, (double)rmin, seen.rho);
       fsg::atomic_min_nan<real>(&a.diag->p_min, (double)pmin, seen.p);
       if (nf) atomicAdd(&a.diag->nonfinite, 1u);
