@@ -170,7 +170,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int HID = 128;
 constexpr int NIN = 14;
 constexpr int NOUT = 11;
-constexpr int K1 = 32;           // padded input width
 constexpr int PROJ_ITERS = 32;   // damped dual Newton iterations, at most
 // Packed bf16 weights (A-operand order, see pack_weights in the host code):
 //   W1: 8 out-tiles x 1 k-step, W2/W3: 8 x 4, W4: 1 x 4 ; each (tile, step)

@@ -1034,7 +1034,6 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
   const int lane = threadIdx.x;
   int64_t e;
   const int b = locate(a, g0 + (int64_t)blockIdx.x, &e);
-  const BlockT<real>& B = a.blocks[b];
   const TypeT<real>& T = a.types[a.btype[b]];
   const int N = T.N;
   const real* src = mode == 0 ? a.det_src[b] : a.work[b];
