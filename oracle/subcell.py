@@ -15,6 +15,11 @@ Sub-faces (a quad face cut into two triangles, which the reference mesh
 rejects) pair each of the triangle side's 16 boundary subfaces with the quad
 side's boundary subface containing its centroid; normal and area come from
 the triangle side (extension, parity unpinned beyond conservation).
+
+The TVD and WENO reconstructions are pinned by closed forms on the inner
+subcells of a hexahedron (exact Green-Gauss gradients for linear and
+quadratic data) and by the TVD trace bound on random data
+(tests/test_oracle_mood.py).
 """
 
 import numpy as np

@@ -202,3 +202,172 @@ def test_positivity_limiter_invariants():
         assert np.all(W[..., 0].min(1) >= rho_floor * (1 - 1e-12))
         # bisection to 2^-30: the pressure bound holds at the accepted theta
         assert np.all(W[..., 4].min(1) >= p_floor * (1 - 1e-6))
+
+
+def _interior_faces(fv, t):
+    """Internal faces between two subcells whose faces are all internal (the
+    central 2 x 2 x 2 block of a 4 x 4 x 4 hexahedron subdivision): there the
+    Green-Gauss gradient over internal faces is exact for linear data."""
+    rs = fv.references[t]
+    o, nb = rs.internal_owner, rs.internal_neighbor
+    nface = np.bincount(np.concatenate([o, nb]), minlength=rs.count)
+    inner = nface == 6
+    return np.flatnonzero(inner[o] & inner[nb])
+
+
+def _linear_subcell_averages(fv, t):
+    c = fv.projections[t].physical_centers
+    X, Y, Z = c[..., 0], c[..., 1], c[..., 2]
+    return np.stack([1.0 + 0.6 * X - 0.3 * Y, 0.3 + 0.4 * Z, -0.2 + 0 * X, 0.1 + 0 * X,
+                     3.0 + 0.5 * Z + 0.25 * X], -1)
+
+
+def test_tvd_reconstruction_linear_data_closed_form():
+    """Linear subcell data on the inner faces: the gradient is exact, so the
+    slope ratio is r = 1 and the limiter phi(r) = (r^2 + 2r) / (r^2 + r + 2)
+    of hybrid/subcell.py:600-615 is 3/4: each side's trace is its own value
+    moved 3/8 of the jump towards the other."""
+    op = _op(2, split=0.0)
+    fv = SubcellFV(op, reconstruction="tvd")
+    t = "hexahedron"
+    f = _interior_faces(fv, t)
+    assert len(f) == 12
+    U = _linear_subcell_averages(fv, t)
+    UL, UR = fv._reconstruct(t, U)
+    o, nb = fv.references[t].internal_owner[f], fv.references[t].internal_neighbor[f]
+    dU = U[:, nb] - U[:, o]
+    assert np.abs(dU).max() > 1e-2
+    # the ratio's 1e-12 regulariser moves r off 1 by 1e-12 / dU: 2e-13 here
+    np.testing.assert_allclose(UL[:, f], U[:, o] + 0.375 * dU, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(UR[:, f], U[:, nb] - 0.375 * dU, rtol=0, atol=1e-11)
+
+
+def test_tvd_reconstruction_stays_between_value_and_midpoint():
+    """For any data, 0 <= phi(r) <= 1 and phi(r) / r <= 1 for r > 0 (and
+    phi = 0 at extrema, r <= 0), so each trace lies between its own subcell
+    value and the mean of the two (checked on the unclamped density and
+    momentum, random data with extrema)."""
+    op = _op(2, split=0.4)
+    fv = SubcellFV(op, reconstruction="tvd")
+    rng = np.random.default_rng(5)
+    for t, rs in fv.references.items():
+        ne = len(fv.projections[t].volumes)
+        U = np.empty((ne, rs.count, 5))
+        U[..., 0] = 1.0 + 0.5 * rng.random((ne, rs.count))
+        U[..., 1:4] = rng.standard_normal((ne, rs.count, 3))
+        U[..., 4] = 20.0 + rng.random((ne, rs.count))
+        UL, UR = fv._reconstruct(t, U)
+        o, nb = rs.internal_owner, rs.internal_neighbor
+        mid = 0.5 * (U[:, o] + U[:, nb])
+        tol = 1e-14
+        for side, own in ((UL, U[:, o]), (UR, U[:, nb])):
+            lo, hi = np.minimum(own, mid), np.maximum(own, mid)
+            assert np.all(side[..., :4] >= lo[..., :4] - tol), t
+            assert np.all(side[..., :4] <= hi[..., :4] + tol), t
+        # some traces are really limited (extrema) and some really moved
+        assert np.any(np.abs(UL[..., 0] - U[:, o, 0]) < 1e-15)
+        assert np.any(np.abs(UL[..., 0] - U[:, o, 0]) > 1e-3)
+
+
+def test_weno_reconstruction_linear_data_closed_form():
+    """Linear subcell data on the inner faces: both one-sided slopes and the
+    jump are equal (s_o = s_n = dU), so the three candidates of
+    hybrid/subcell.py:617-645 are the midpoint (central and one-sided) and
+    the cell value, with smoothness indicators beta_h = beta_o = dU^2 / 10,
+    beta_c = dU^2 and tau = 9 dU^2 / 10.  The weights are then
+    0.70 (1 + 81), 0.28 (1 + 81) and 0.02 (1 + 0.81): the trace is the
+    cell value moved (1 - w) / 2 of the jump, w = 0.0362 / 80.3962."""
+    op = _op(2, split=0.0)
+    fv = SubcellFV(op, reconstruction="weno")
+    t = "hexahedron"
+    f = _interior_faces(fv, t)
+    U = _linear_subcell_averages(fv, t)
+    UL, UR = fv._reconstruct(t, U)
+    o, nb = fv.references[t].internal_owner[f], fv.references[t].internal_neighbor[f]
+    dU = U[:, nb] - U[:, o]
+    w = 0.02 * 1.81 / (0.70 * 82 + 0.28 * 82 + 0.02 * 1.81)
+    big = np.abs(dU) > 1e-3        # the 1e-12 regulariser is negligible there
+    assert big.sum() > 100
+    want_l = U[:, o] + 0.5 * (1 - w) * dU
+    want_r = U[:, nb] - 0.5 * (1 - w) * dU
+    assert np.abs((UL[:, f] - want_l)[big]).max() < 1e-12
+    assert np.abs((UR[:, f] - want_r)[big]).max() < 1e-12
+    # components without a jump keep the cell value
+    assert np.abs((UL[:, f] - U[:, o])[np.abs(dU) < 1e-14]).max() < 1e-14
+
+
+def test_tvd_reconstruction_quadratic_data_closed_form():
+    """rho = 1 + (x + 1)^2 on the inner faces: the Green-Gauss gradient at an
+    inner subcell centre is exact for a quadratic on the uniform subdivision
+    (2 (x_c + 1)), while the jump to the neighbour is not the one-sided slope
+    times the distance, so r = dU / (g_o . dc) != 1 and the trace is
+    U_o + phi(r) g_o . dc / 2 with the limiter of hybrid/subcell.py:600-615,
+    not a multiple of the jump."""
+    op = _op(2, split=0.0)
+    fv = SubcellFV(op, reconstruction="tvd")
+    t = "hexahedron"
+    f = _interior_faces(fv, t)
+    c = fv.projections[t].physical_centers
+    U = np.zeros(c.shape[:2] + (5,))
+    U[..., 0] = 1.0 + (c[..., 0] + 1.0) ** 2
+    U[..., 4] = 3.0
+    UL, UR = fv._reconstruct(t, U)
+    o, nb = fv.references[t].internal_owner[f], fv.references[t].internal_neighbor[f]
+    dx = c[:, nb, 0] - c[:, o, 0]
+    along = np.abs(dx) > 1e-9          # faces normal to x (4 of the 12)
+    assert along.sum() == 4 * len(c)
+
+    def phi(r):
+        return np.clip(np.where(r > 0, (r * r + 2 * r) / (r * r + r + 2), 0.0), 0.0, 1.0)
+
+    dU = U[:, nb, 0] - U[:, o, 0]
+    so = 2 * (c[:, o, 0] + 1) * dx         # exact gradient at the owner centre . dc
+    sn = -2 * (c[:, nb, 0] + 1) * dx       # and at the neighbour, towards the owner
+    with np.errstate(invalid="ignore", divide="ignore"):   # faces not normal to x: 0 / 0
+        want_l = U[:, o, 0] + 0.5 * phi(dU / so) * so
+        want_r = U[:, nb, 0] + 0.5 * phi(-dU / sn) * sn
+        assert np.abs(dU / so - 1)[along].min() > 1e-2
+    assert np.abs(UL[:, f, 0] - want_l)[along].max() < 1e-10
+    assert np.abs(UR[:, f, 0] - want_r)[along].max() < 1e-10
+    assert np.abs(UL[:, f, 0] - U[:, o, 0])[~along].max() == 0.0
+
+
+def test_weno_reconstruction_quadratic_data_closed_form():
+    """rho = 1 + (x + 1)^2 on the inner faces normal to x: exact one-sided
+    slopes s_o = 2 (x_o + 1) dx and s_n = 2 (x_n + 1) dx (Green-Gauss is
+    exact there), secant dU = s_o + dx^2, all three different.  The trace is
+    the CWENO-Z combination of hybrid/subcell.py:604-638 (central Hermite
+    value, one-sided value, cell value; linear weights 0.70 / 0.28 / 0.02;
+    beta from the slope differences, tau = |beta_c - beta_h|), written here
+    from those lines with the exact slopes."""
+    op = _op(2, split=0.0)
+    fv = SubcellFV(op, reconstruction="weno")
+    t = "hexahedron"
+    f = _interior_faces(fv, t)
+    c = fv.projections[t].physical_centers
+    U = np.zeros(c.shape[:2] + (5,))
+    U[..., 0] = 1.0 + (c[..., 0] + 1.0) ** 2
+    U[..., 4] = 3.0
+    UL, UR = fv._reconstruct(t, U)
+    o, nb = fv.references[t].internal_owner[f], fv.references[t].internal_neighbor[f]
+    dx = c[:, nb, 0] - c[:, o, 0]
+    along = np.abs(dx) > 1e-9
+    uo, un = U[:, o, 0], U[:, nb, 0]
+    so = 2 * (c[:, o, 0] + 1) * dx
+    sn = 2 * (c[:, nb, 0] + 1) * dx
+    sec = un - uo
+    e = 1e-12
+    bh = (sn - so) ** 2 + 0.1 * sec ** 2
+    bo = (sec - so) ** 2 + 0.1 * sec ** 2
+    bn = (sec - sn) ** 2 + 0.1 * sec ** 2
+    bc = sec ** 2
+    tau = np.abs(bc - bh)
+    central = 0.5 * (uo + un) + 0.125 * (so - sn)
+    w = [0.70 * (1 + (tau / (bh + e)) ** 2), 0.28 * (1 + (tau / (bo + e)) ** 2),
+         0.02 * (1 + (tau / (bc + e)) ** 2)]
+    want_l = (w[0] * central + w[1] * (uo + 0.5 * so) + w[2] * uo) / (w[0] + w[1] + w[2])
+    w[1] = 0.28 * (1 + (tau / (bn + e)) ** 2)
+    want_r = (w[0] * central + w[1] * (un - 0.5 * sn) + w[2] * un) / (w[0] + w[1] + w[2])
+    assert np.abs(sec - so)[along].min() > 1e-4 and np.abs(sn - so)[along].min() > 1e-4
+    assert np.abs(UL[:, f, 0] - want_l)[along].max() < 1e-12
+    assert np.abs(UR[:, f, 0] - want_r)[along].max() < 1e-12
