@@ -27,7 +27,9 @@ Pinning status of each oracle function (what checks it, besides retyping):
   (tests/test_air11_chemistry.py, tests/test_reacting_coupler.py), detailed
   balance, finite-difference Jacobian.
 * limiter / subcell: P1 field -> sensor 0, closed-form single-node ratio,
-  reconstruct∘project = I, conservation of recompute to round-off.
+  reconstruct∘project = I, conservation of recompute to round-off, TVD and
+  WENO traces in closed form where the subcell gradient is exact, the TVD
+  trace bound.
 
 Functions without such a check are marked "parity unpinned" in their docstring.
 """
