@@ -371,3 +371,36 @@ def test_weno_reconstruction_quadratic_data_closed_form():
     assert np.abs(sec - so)[along].min() > 1e-4 and np.abs(sn - so)[along].min() > 1e-4
     assert np.abs(UL[:, f, 0] - want_l)[along].max() < 1e-12
     assert np.abs(UR[:, f, 0] - want_r)[along].max() < 1e-12
+
+
+def test_positivity_limiter_is_tight():
+    """The limiter (hybrid/limiter.py:100-147) scales the deviations from the
+    mean by the largest theta that keeps every sample admissible: theta_rho
+    in closed form puts the lowest sampled density exactly on its floor, and
+    the bisection on theta_p leaves the lowest sampled pressure within the
+    bisection's resolution of its floor.  So every limited element ends with
+    one of the two at its floor (a limiter that shrinks further, to the mean
+    say, satisfies the floors too but fails this)."""
+    op = _op(2)
+    pos = PositivityLimiter(op, density_floor=1e-6, pressure_floor=1e-6)
+    st = _states(op, lambda x: perturbed_primitive(x, seed=4))
+    bad = {}
+    for t, U in st.items():
+        W = op.solver.conserved_to_primitive(U.copy())
+        W[::2, 0, 0] = -0.05
+        W[1::3, 1, 4] = -0.02
+        bad[t] = op.solver.primitive_to_conserved(W)
+    out = pos(bad)
+    limited = 0
+    for bid, b in enumerate(op.mesh.element_blocks):
+        t = b.element_type
+        Wm = op.solver.conserved_to_primitive(pos.mean(bid, bad[t]))
+        W = op.solver.conserved_to_primitive(pos.sample(t, out[t]))
+        rho_floor = np.maximum(1e-6, 0.05 * Wm[:, 0])
+        p_floor = np.maximum(1e-6, 0.05 * Wm[:, 4])
+        changed = np.abs(out[t] - bad[t]).max((1, 2)) > 1e-12
+        on_rho = np.abs(W[..., 0].min(1) - rho_floor) <= 1e-10 * rho_floor
+        on_p = (W[..., 4].min(1) - p_floor) <= 1e-6 * Wm[:, 4]
+        assert np.all((on_rho | on_p)[changed]), t
+        limited += changed.sum()
+    assert limited > 10
