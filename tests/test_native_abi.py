@@ -66,3 +66,30 @@ def test_support_tables_are_face_local():
                 off = np.setdiff1d(np.arange(phi.shape[1]), sup)
                 assert np.abs(phi[:, off]).max() < 1e-13
                 assert len(sup) == ((p + 1) ** 2 if ft == "quad" else (p + 1) * (p + 2) // 2)
+
+
+def test_null_arguments_are_rejected_before_any_device_work():
+    """Every entry point checks its pointers first and reports through its
+    module's last-error string (runs without a GPU: nothing is launched)."""
+    import ctypes
+    from conftest import PLANE_NAMES
+    from hybrid import HybridCompressibleSolver, HybridNavierStokesOperator, box_mesh
+    from hybrid import _native as nat
+    lib = nat.lib()
+    assert lib.fsg_diag_reset(None, None) == -10
+    assert b"null" in lib.fsg_last_error()
+    assert lib.fsg_residual_masked(None, None, None, 0, 0, None, None, None) == -10
+    assert lib.fsg_mood_positivity(None, None, None, None) == -10
+    assert lib.fsg_chem_integrate(None, None, None, None, 1e-7, 12, 0, None, None, 0, None) == -10
+    assert b"null" in lib.fsg_chem_last_error()
+    assert lib.fsg_mlp_adam(None, None, None, None, 0, 1e-3, 0.9, 0.999, 1e-8, 1, None) == -10
+    assert b"null" in lib.fsg_mlp_last_error()
+    # the masked residual names the block whose mask is missing
+    m = box_mesh(3, split_fraction=0.4, seed=3, plane_names=PLANE_NAMES)
+    s = HybridCompressibleSolver(m, polynomial_degree=2, dynamic_viscosity=0.01, device="cpu")
+    dm = HybridNavierStokesOperator(s).device_mesh
+    nb = len(dm.block_tables)
+    arr = (ctypes.c_void_p * nb)()
+    masks = (ctypes.c_void_p * nb)()
+    rc = lib.fsg_residual_masked(dm.ctx, arr, arr, 0, 3, masks, None, None)
+    assert rc == -10 and b"null mask of block 0" in lib.fsg_last_error()
