@@ -41,6 +41,9 @@ def test_context_over_cpu_tables_and_argument_checks():
     f = s.mesh.faces
     nrec = sum(int(b["rec_ptr"][-1]) for b in dm.block_tables)
     assert nrec == 2 * int((f.neighbor_block >= 0).sum()) + int((f.neighbor_block < 0).sum())
+    # a record holds one face's support nodes, at most (p + 1)^2: the element
+    # kernels' prefetched record registers are sized for that
+    assert max(int(b["rec_nf"].max()) for b in dm.block_tables) <= (2 + 1) ** 2
     # unsupported configuration is rejected with a message, not a crash
     p = nat.Params()
     p.degree, p.nvars, p.dtype = 4, 5, 0
