@@ -424,7 +424,10 @@ int fsg_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float 
  * re-projected onto the input's N, O, charge and mass content, into rho_out;
  * cells outside the hull or whose projection misses the tolerance get
  * fallback[c] = 1 and rho_out = rho_in, to be advanced by fsg_chem_integrate.
- * blocks <= 0 launches one block per 64 cells. */
+ * rho_out must not overlap rho_in (the raw network outputs are staged in
+ * rho_out before the projection reads rho_in; error -11).  A block holds 16
+ * waves of 64 cells: blocks <= 0 launches ceil(n / 1024) blocks, a positive
+ * value caps the grid (the waves then stride over the cells). */
 int32_t fsg_mlp_packed_weight_count(void);
 int32_t fsg_mlp_consts_size(void);
 int fsg_mlp_infer(const void* packed_weights, const float* biases, const void* consts,

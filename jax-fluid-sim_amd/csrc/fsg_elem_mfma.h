@@ -77,6 +77,13 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
   DiagSeen seen{};
   if (a.mode == 2 && lane == 0) seen = diag_seen(a.diag);
   const int ne = (int)(count - e0 < EB ? count - e0 : EB);
+  if (a.mask) {
+    // masked evaluation: a batch without a marked element has nothing to do
+    // (the decision is uniform over the workgroup, before any barrier)
+    bool any = false;
+    for (int e = 0; e < ne; ++e) any |= a.mask[e0 + e] != 0;
+    if (!any) return;
+  }
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;
@@ -365,6 +372,7 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
     real* ob = a.out + e0 * (N * V);
     for (int k = tid; k < ne * N * V; k += 256) {
       const int e = k / (N * V), r = k - e * (N * V), i = r / V, v = r - i * V;
+      if (a.mask && a.mask[e0 + e] == 0) continue;
       ob[k] = sR[i * WU + e * V + v];
     }
     return;
@@ -400,7 +408,7 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
         const real rhs = sS[e] * div[g][j];
         const int64_t gi = ((e0 + e) * N + i) * V + v;
         if (a.mode == 1) {
-          a.out[gi] = rhs;
+          if (!a.mask || a.mask[e0 + e] != 0) a.out[gi] = rhs;
         } else {
           const real dte = a.dt_elem ? a.dt_elem[e0 + e] : a.dt;
           const real nxt = a.alpha * a.Ubase[gi] + a.beta * (sU[i * WU + col] + dte * rhs);

@@ -624,6 +624,14 @@ int fsg_mlp_infer(const void* packed_weights, const float* biases, const void* c
   if (!packed_weights || !biases || !consts || !rho_in || !T_in || !rho_out || !fallback)
     return mlp_fail(-10, "null argument");
   if (n <= 0) return 0;
+  // the network's raw outputs are staged in rho_out before the projection
+  // reads rho_in again: the two must not overlap
+  {
+    const char* a = reinterpret_cast<const char*>(rho_in);
+    const char* b = reinterpret_cast<const char*>(rho_out);
+    const int64_t bytes = n * fsgmlp::NOUT * (int64_t)sizeof(double);
+    if (a < b + bytes && b < a + bytes) return mlp_fail(-11, "rho_out overlaps rho_in");
+  }
   fsgmlp::InferConsts c;
   memcpy(&c, consts, sizeof c);
   int64_t groups = (n + 63) / 64;

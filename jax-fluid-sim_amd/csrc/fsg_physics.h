@@ -407,16 +407,17 @@ __device__ __forceinline__ void ghost_viscous(const real* Uin, const real* n, in
                                               const Phys<real>& ph, real* vg) {
   viscous_variables<V, real>(Uin, ph.gamma, ph.gas_constant, vg);
   if (!ph.has_boundary) return;
-  if (flags & BND_NOSLIP) {
-    vg[0] = -vg[0];
-    vg[1] = -vg[1];
-    vg[2] = -vg[2];
-    if (ph.has_wall_temperature) vg[3] = real(2) * ph.wall_temperature - vg[3];
-  }
+  // The slip ghost replaces the no-slip one when a tag is in both sets: it
+  // reflects the inside velocity and keeps the inside temperature.
   if (flags & BND_SLIP) {
     real un = vg[0] * n[0] + vg[1] * n[1] + vg[2] * n[2];
 #pragma unroll
     for (int d = 0; d < 3; ++d) vg[d] = vg[d] - real(2) * un * n[d];
+  } else if (flags & BND_NOSLIP) {
+    vg[0] = -vg[0];
+    vg[1] = -vg[1];
+    vg[2] = -vg[2];
+    if (ph.has_wall_temperature) vg[3] = real(2) * ph.wall_temperature - vg[3];
   }
   if (flags & BND_INLET) {
     real Uf[V];

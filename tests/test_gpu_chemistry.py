@@ -13,7 +13,15 @@ test also asserts that most cells sit at the absolute 1e-10 bar (short
 Newton counts, constant T) so the comparison pins the arithmetic.  The
 invariants (species clip, total-mass rescale, flow variables untouched) are
 checked on every cell at round-off.  Cells whose floor exceeds 1e-8 are not
-compared at all (a few noisy trials do not bound a chaotic spread)."""
+compared at all (a few noisy trials do not bound a chaotic spread); every
+test prints and asserts the fraction of cells that are compared.
+
+On the random far-from-equilibrium batch, 12 coupled iterations leave no cell
+below that floor, so that case checks the invariants only.  The coupled
+12-iteration scheme that the reacting flow runs is pinned on the states where
+it is well conditioned: air (N2/O2) at 1500-3000 K, the reacting flow's
+inflow regime, and cells first brought to their isothermal equilibrium at
+3000-5500 K.  There every cell must match at the absolute 1e-10 bar."""
 
 import numpy as np
 import pytest
@@ -45,7 +53,8 @@ class NoisyMechanism(Mechanism):
 
 
 def _bars(ref_fn, refs, scale, trials=3):
-    """Per-cell bars max(TOL, 4 floor) * scale from re-running the oracle."""
+    """Per-cell bars max(TOL, 4 floor) * scale from re-running the oracle
+    (infinite where the floor exceeds UNPINNED)."""
     floors = np.zeros(len(refs))
     with np.errstate(all="ignore"):
         for j in range(trials):
@@ -61,10 +70,45 @@ def _bars(ref_fn, refs, scale, trials=3):
     return np.maximum(TOL, 4.0 * floors) * scale
 
 
-@pytest.mark.parametrize("const_T,iters,pinned", [(True, 12, 0.95), (False, 1, 0.9),
-                                                  (False, 2, 0.6), (False, 12, 0.0)])
-def test_integrator_matches_oracle(const_T, iters, pinned):
-    rho, T = make_batch(96, seed=3)
+def air_batch(n, seed, T_range=(1500.0, 3000.0)):
+    """Air (N2 0.767, O2 0.233 by mass) at T uniform in T_range and p
+    log-uniform in [1e3, 1e5] Pa."""
+    rng = np.random.default_rng(seed)
+    om = Mechanism()
+    T = rng.uniform(*T_range, n)
+    p = np.exp(rng.uniform(np.log(1e3), np.log(1e5), n))
+    Y = np.zeros((n, 11))
+    Y[:, 9], Y[:, 10] = 0.767, 0.233
+    R = 8.31446261815324 * np.sum(Y / om.molar_masses, 1)
+    return (p / (R * T))[:, None] * Y, T
+
+
+def equilibrated_batch(n, seed, T_range=(3000.0, 5500.0)):
+    """make_batch cells brought close to their isothermal equilibrium by the
+    oracle's backward Euler with a long step (dt 1e-3 s, 60 iterations)."""
+    rho, T = make_batch(n, seed=seed, T_range=T_range)
+    om = Mechanism()
+    return np.array([integrator_step(om, rho[c], T[c], 1e-3, 60, True)[0]
+                     for c in range(n)]), T
+
+
+def _report(name, bars, scale):
+    finite = float(np.mean(np.isfinite(bars)))
+    pinned = float(np.mean(bars <= TOL * scale * (1 + 1e-12)))
+    print(f"{name}: compared {finite:.3f}, at the 1e-10 bar {pinned:.3f}", flush=True)
+    return finite, pinned
+
+
+@pytest.mark.parametrize("batch,const_T,iters,pinned", [
+    ("random", True, 12, 0.95), ("random", False, 1, 0.9), ("random", False, 2, 0.6),
+    ("random", False, 12, 0.0), ("air", False, 12, 1.0), ("equilibrated", False, 12, 1.0)])
+def test_integrator_matches_oracle(batch, const_T, iters, pinned):
+    if batch == "random":
+        rho, T = make_batch(96, seed=3)
+    elif batch == "air":
+        rho, T = air_batch(48, seed=3)
+    else:
+        rho, T = equilibrated_batch(24, seed=5)
     mech = Air11ParkMechanism()
     integ = PointImplicitIntegrator(mech, max_newton_iters=iters)
     got, Tg, _ = integ.step(rho, T, 1e-7, constant_temperature=const_T)
@@ -79,7 +123,10 @@ def test_integrator_matches_oracle(const_T, iters, pinned):
     err = np.array([np.abs(got[c] - refs[c]).max() for c in range(len(T))])
     bad = np.nonzero(err > bars)[0]
     assert bad.size == 0, [(int(c), float(err[c]), float(bars[c])) for c in bad[:5]]
-    assert np.mean(bars <= TOL * scale * (1 + 1e-12)) >= pinned
+    finite, at_bar = _report(f"integrator {batch} const_T={const_T} iters={iters}", bars, scale)
+    assert at_bar >= pinned
+    if batch != "random":
+        assert finite == 1.0 and at_bar == 1.0
     # Invariants: clip, total-mass rescale, temperature bounds.
     assert np.all(got >= 0) and np.all(np.isfinite(got))
     np.testing.assert_allclose(got.sum(1), np.maximum(rho, 1e-20).sum(1), rtol=1e-13)
@@ -89,11 +136,11 @@ def test_integrator_matches_oracle(const_T, iters, pinned):
         assert np.all((Tg >= 100.0) & (Tg <= 50000.0))
 
 
-def _random_block(dimensionless, seed=7, E=6, N=10):
-    """Charge-neutral air batch (T 3000-12000 K, p 1e3-1e5 Pa) as flow nodes
-    with random velocities up to 1500 m/s; non-dimensionalised with the
-    coupler's reference scales when asked."""
-    rho_s, T = make_batch(E * N, seed=seed)
+def _random_block(dimensionless, seed=7, E=6, N=10, air=False):
+    """Charge-neutral air batch (T 3000-12000 K, p 1e3-1e5 Pa; or N2/O2 air at
+    1500-3000 K) as flow nodes with random velocities up to 1500 m/s;
+    non-dimensionalised with the coupler's reference scales when asked."""
+    rho_s, T = air_batch(E * N, seed) if air else make_batch(E * N, seed=seed)
     rng = np.random.default_rng(seed)
     om = Mechanism()
     c = OracleCoupler(om, dimensionless=True)
@@ -129,7 +176,8 @@ def _check_coupler(U, dt, iters, dimensionless, pinned):
     err = np.abs(got - ref).reshape(len(refs), -1).max(1)
     bad = np.nonzero(err > bars)[0]
     assert bad.size == 0, [(int(c), float(err[c]), float(bars[c])) for c in bad[:5]]
-    assert np.mean(bars <= TOL * scale * (1 + 1e-12)) >= pinned
+    _, at_bar = _report(f"coupler iters={iters} dimensionless={dimensionless}", bars, scale)
+    assert at_bar >= pinned
     # hybrid/chemistry/coupling.py:170-177: flow variables untouched, species
     # non-negative and summing to rho.
     assert np.array_equal(got[..., :5], U[..., :5])
@@ -139,9 +187,10 @@ def _check_coupler(U, dt, iters, dimensionless, pinned):
 
 
 @pytest.mark.parametrize("dimensionless", [False, True])
-@pytest.mark.parametrize("iters,pinned", [(1, 0.85), (2, 0.55), (12, 0.0)])
-def test_coupler_matches_oracle(dimensionless, iters, pinned):
-    U = _random_block(dimensionless)
+@pytest.mark.parametrize("iters,pinned,air", [(1, 0.85, False), (2, 0.55, False),
+                                              (12, 0.0, False), (12, 1.0, True)])
+def test_coupler_matches_oracle(dimensionless, iters, pinned, air):
+    U = _random_block(dimensionless, air=air)
     dt = 1e-7 if not dimensionless else 1e-7 / OracleCoupler(dimensionless=True).t_ref
     _check_coupler(U, dt, iters, dimensionless, pinned)
 

@@ -44,6 +44,16 @@ def shock_primitive(x, seed=0):
     return W
 
 
+def _unit(name):
+    """Coordinate factor that maps the case's box onto the unit box the
+    shock fields are defined on."""
+    return 10.0 if name == "sliver" else 1.0
+
+
+def _dt(name):
+    return 2e-4 if name == "sliver" else 2e-3
+
+
 def _case(name, p, recon="weno", indexing="reference", dtype=torch.float64):
     boundary = name != "periodic"
     if name == "periodic":
@@ -52,6 +62,9 @@ def _case(name, p, recon="weno", indexing="reference", dtype=torch.float64):
         m = box_mesh(4, split_fraction=0.4, seed=7, plane_names=PLANE_NAMES)
     elif name == "split_periodic":
         m = box_mesh(4, periodic=(True,) * 3, split_fraction=0.5, seed=3)
+    elif name == "sliver":
+        # the split box scaled to side 0.1: M^-1 carries min(1, V/1e-4) < 1
+        m = box_mesh(4, lengths=(0.1,) * 3, split_fraction=0.4, seed=7, plane_names=PLANE_NAMES)
     else:
         raise KeyError(name)
     mu = 0.005
@@ -60,7 +73,7 @@ def _case(name, p, recon="weno", indexing="reference", dtype=torch.float64):
     op = HybridNavierStokesOperator(solver)
     osolver = CompressibleSolver(gamma=1.4, dynamic_viscosity=mu, prandtl=0.71)
     oop = NavierStokesOperator(from_arrays(m), osolver, p)
-    states = {t: osolver.primitive_to_conserved(shock_primitive(x))
+    states = {t: osolver.primitive_to_conserved(shock_primitive(x * _unit(name)))
               for t, x in oop.geometry.solution_node_coordinates.items()}
     b = ob = None
     if boundary:
@@ -94,7 +107,7 @@ def _bar(fn, states, tol=TOL):
     return np.maximum(tol, 4.0 * worst)
 
 
-CASES = [("periodic", 2), ("split", 2), ("split", 3), ("split_periodic", 2)]
+CASES = [("periodic", 2), ("split", 2), ("split", 3), ("split_periodic", 2), ("sliver", 2)]
 
 
 @pytest.mark.parametrize("name,p", CASES)
@@ -133,10 +146,11 @@ def test_positivity_matches(name, p):
 
 @pytest.mark.parametrize("recon", ["constant", "tvd", "weno"])
 @pytest.mark.parametrize("indexing", ["reference", "owner"])
-@pytest.mark.parametrize("name,p", [("split", 2), ("split_periodic", 2), ("split", 3)])
+@pytest.mark.parametrize("name,p", [("split", 2), ("split_periodic", 2), ("split", 3),
+                                    ("sliver", 2)])
 def test_recompute_matches(name, p, recon, indexing):
     solver, op, b, lim, oop, ob, olim, states = _case(name, p, recon, indexing)
-    dt = 2e-3
+    dt = _dt(name)
     inv = oop.inviscid_rhs(states, ob)
     visc = oop.viscous_rhs(states, ob)
     cand = {k: states[k] + dt * (inv[k] + visc[k]) for k in states}
@@ -177,16 +191,16 @@ def test_owner_indexing_conserves(recon):
 
 
 @pytest.mark.parametrize("fused", [True, False])
-@pytest.mark.parametrize("name,p", [("split", 2), ("periodic", 2), ("split", 3)])
+@pytest.mark.parametrize("name,p", [("split", 2), ("periodic", 2), ("split", 3), ("sliver", 2)])
 def test_mood_stage_matches(name, p, fused):
     """Both stage forms against the oracle: one residual pass with the
     viscous rhs re-evaluated on the troubled elements (fused), and the
     reference's separate inviscid and viscous passes."""
     print(f"case {name} p={p} fused={fused}", flush=True)
     solver, op, b, lim, oop, ob, olim, states = _case(name, p)
-    curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x, seed=9))
+    curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x * _unit(name), seed=9))
             for t, x in oop.geometry.solution_node_coordinates.items()}
-    dt = 2e-3
+    dt = _dt(name)
     rk = SSPRK3(oop, limiter=olim)
 
     def oracle(c):
@@ -214,7 +228,7 @@ def test_checked_step_with_mood():
 FP32_TOL = 1e-4
 
 
-@pytest.mark.parametrize("name,p", [("split", 2), ("split_periodic", 2)])
+@pytest.mark.parametrize("name,p", [("split", 2), ("split_periodic", 2), ("sliver", 2)])
 def test_fp32_path(name, p):
     """FP32 MOOD kernels: identical troubled sets (every sensor value is at
     least 4e-4 away from the threshold on these states, far above FP32
@@ -227,9 +241,9 @@ def test_fp32_path(name, p):
     got = lim.troubled(_dev(solver, states))
     for t in want:
         assert np.array_equal(got[t].cpu().numpy(), want[t]), t
-    curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x, seed=9))
+    curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x * _unit(name), seed=9))
             for t, x in oop.geometry.solution_node_coordinates.items()}
-    dt = 2e-3
+    dt = _dt(name)
     w, wv, _, _ = SSPRK3(oop, limiter=olim).rk_stage(states, curr, dt, 0.75, 0.25, ob)
     g, gv, _, _ = HybridSSPRK3(op, limiter=lim).rk_stage(_dev(solver, states), _dev(solver, curr),
                                                          dt, 0.75, 0.25, b)

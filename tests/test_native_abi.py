@@ -96,3 +96,20 @@ def test_null_arguments_are_rejected_before_any_device_work():
     masks = (ctypes.c_void_p * nb)()
     rc = lib.fsg_residual_masked(dm.ctx, arr, arr, 0, 3, masks, None, None)
     assert rc == -10 and b"null mask of block 0" in lib.fsg_last_error()
+
+
+def test_surrogate_inference_rejects_overlapping_output():
+    """fsg_mlp_infer stages the network outputs in rho_out before it reads
+    rho_in again: overlapping buffers are refused before any launch."""
+    import ctypes
+    from hybrid import _native as nat
+    lib = nat.lib()
+    n = 100
+    buf = (ctypes.c_double * (2 * n * 11))()
+    base = ctypes.addressof(buf)
+    one = ctypes.c_void_p(base)
+    for off in (0, 8 * 11 * (n - 1)):      # same buffer, and a partial overlap
+        rc = lib.fsg_mlp_infer(one, one, one, ctypes.c_void_p(base), one, None,
+                               ctypes.c_double(1e-7), ctypes.c_void_p(base + off), one,
+                               ctypes.c_int64(n), 0, None)
+        assert rc == -11 and b"overlaps" in lib.fsg_mlp_last_error()
