@@ -375,6 +375,22 @@ int fsg_chem_integrate_flagged(const double* table, const double* rho_in, const 
                                int32_t* count, double* rho_out, double* T_out, int64_t n,
                                void* stream);
 
+/* fsg_chem_integrate_flagged without caller scratch and without any device
+ * allocation: the flagged-cell list and its counter live in a static device
+ * array of the library (one per GPU, FSG_NOALLOC_MAX_CELLS + 64 int32 =
+ * 8 MiB, kept until the process ends).  No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, serving loops).
+ * n above FSG_NOALLOC_MAX_CELLS is rejected (error -13).  The counter is
+ * cleared by a fill kernel at each call; calls that overlap on one GPU (two
+ * streams, or two host threads) share the array and race.  Same per-cell
+ * result as fsg_chem_integrate_flagged. */
+#define FSG_NOALLOC_MAX_CELLS (1 << 21)
+int fsg_chem_integrate_flagged_noalloc(const double* table, const double* rho_in,
+                                       const double* T_in, const double* dt_cell, double dt,
+                                       int32_t newton_iters, int32_t constant_temperature,
+                                       const int32_t* flag, double* rho_out, double* T_out,
+                                       int64_t n, void* stream);
+
 /* The coupler's steps around its integrator call, for another substep in
  * between (the surrogate): prepare = species clip and rescale to rho, thermal
  * energy, T0 recovery (hybrid/chemistry/coupling.py:116-154) giving the

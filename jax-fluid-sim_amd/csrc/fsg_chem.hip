@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/fluid_gfx950.h"
@@ -503,6 +504,12 @@ __global__ __launch_bounds__(64) void integrate_list_kernel(
 
 }  // namespace fsgchem
 
+// Scratch of fsg_chem_integrate_flagged_noalloc: the compacted list of
+// flagged cells (FSG_NOALLOC_MAX_CELLS int32) followed by its counter.  One
+// copy per GPU, held until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns on each device.
+static __device__ __align__(256) int32_t scratch[FSG_NOALLOC_MAX_CELLS + 64];
+
 static thread_local std::string g_chem_error;
 
 static int chem_fail(int code, const char* msg) {
@@ -564,6 +571,37 @@ int fsg_chem_integrate_flagged(const double* table, const double* rho_in, const 
                      rho_in, T_in, dt_cell, dt, newton_iters, constant_temperature, list, count,
                      rho_out, T_out);
   return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
+}
+
+int fsg_chem_integrate_flagged_noalloc(const double* table, const double* rho_in,
+                                       const double* T_in, const double* dt_cell, double dt,
+                                       int32_t newton_iters, int32_t constant_temperature,
+                                       const int32_t* flag, double* rho_out, double* T_out,
+                                       int64_t n, void* stream) {
+  if (!table || !rho_in || !T_in || !flag || !rho_out) return chem_fail(-10, "null argument");
+  if (n <= 0) return 0;
+  if (n > FSG_NOALLOC_MAX_CELLS)
+    return chem_fail(-13, "batch exceeds FSG_NOALLOC_MAX_CELLS cells of the static scratch");
+  // the scratch address of the current device, looked up once per device
+  static std::mutex lock;
+  static int32_t* base[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return chem_fail(-1, "no current HIP device");
+  int32_t* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> g(lock);
+    if (!base[dev]) {
+      void* addr = nullptr;
+      if (hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)) != hipSuccess || !addr)
+        return chem_fail(-1, "hipGetSymbolAddress of the chemistry scratch failed");
+      base[dev] = static_cast<int32_t*>(addr);
+    }
+    ptr = base[dev];
+  }
+  return fsg_chem_integrate_flagged(table, rho_in, T_in, dt_cell, dt, newton_iters,
+                                    constant_temperature, flag, ptr, ptr + FSG_NOALLOC_MAX_CELLS,
+                                    rho_out, T_out, n, stream);
 }
 
 int fsg_chem_prepare(const double* table, const void* U, int32_t dtype, int64_t ncells,

@@ -125,6 +125,9 @@ def lib():
         L.fsg_chem_integrate_flagged.argtypes = [vp, vp, vp, vp, ctypes.c_double, i32, i32, vp,
                                                  vp, vp, vp, vp, i64, vp]
         L.fsg_chem_integrate_flagged.restype = ctypes.c_int
+        L.fsg_chem_integrate_flagged_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_double, i32,
+                                                         i32, vp, vp, vp, i64, vp]
+        L.fsg_chem_integrate_flagged_noalloc.restype = ctypes.c_int
         L.fsg_chem_prepare.argtypes = [vp, vp, i32, i64, ctypes.POINTER(ChemParams), vp, vp, vp]
         L.fsg_chem_prepare.restype = ctypes.c_int
         L.fsg_chem_finish.argtypes = [vp, i32, i64, ctypes.POINTER(ChemParams), vp, vp]
