@@ -1,20 +1,26 @@
-"""Benchmark of the hybrid DG residual + SSP-RK3 stage on one MI355X.
+"""Benchmark of the hybrid DG hot path on one MI355X.
 
-Workload (the metric's configuration that fits one GPU): Taylor-Green vortex
-on a periodic [0, 2 pi]^3 box of 32^3 hexahedra, DG p=3 (64 nodes/element,
-2,097,152 DOF per variable), 5 conserved variables, gamma 1.4, Pr 0.71,
-Mach 0.1, Re 1600, HLLC + BR2, SSP-RK3 at CFL 0.3, no limiter, no chemistry.
-Synthetic data: the analytic TGV initial condition at the solution nodes.
+Headline workload (the metric's reacting configuration, which fits one
+GPU): the reacting hybrid-mesh flow -- unit box of 86^3 hexahedra with a
+seeded 20 % split into wedges and tetrahedra (1,016,939 elements,
+19.8M nodes), DG p=2, 16 variables (5 flow + 11 species), Mach 3 air at
+1500 K with a seeded 1 % density perturbation, HLLC + BR2 at Re 1e4, MOOD +
+positivity, SSP-RK3 at CFL 0.3, and the Park-11 coupler after every accepted
+step.  Synthetic data: the analytic inflow state at the solution nodes.
 
-A timed step is one full SSP-RK3 step (three fused residual + RK-epilogue
-stages) on state already resident in HBM.  The plain FP64 path and the FP32
-path run the same number of steps from the same initial state with the same
-dt; ``value`` is the DOF-updates/s per RK stage of the FP32 path when its
-relative L2 error against the FP64 path stays within 1e-4 for density and
-energy, else of the FP64 path.  ``roofline`` reports the face kernel (the
-longest launch, bound by HBM latency and bandwidth) timed with HIP events on
-its own stream, ``element_kernel`` the sum-factorised element kernel; ``cpu_baseline``
-times the CPU FP64 oracle on a bounded sample (8^3 elements, 2 stages).
+A timed step is one full SSP-RK3 step (three MOOD stages) plus the chemistry
+substep, on state already resident in HBM.  The plain path (FP64 residual,
+FP64 point-implicit chemistry) and the fast path (FP32 residual, BF16-MFMA
+chemistry surrogate with the FP64 integrator as fallback) run the same steps
+with the same dt; ``value`` is the fast path's DOF-updates/s per RK stage,
+with the plain path's rate, the speed-up and the fast path's relative L2
+error against the plain FP64 path after ``--error-steps`` steps beside it.
+``roofline`` is the fast path's longest-running kernel over the timed
+region (HIP events around each launch on its stream; full residual passes
+only), with its algorithmic work per launch.  ``cpu_baseline`` times the
+CPU FP64 oracle on a bounded sample of the same workload.  Extra keys: the
+Taylor-Green vortex (32^3, p=3), the 0-D chemistry batch (2^20 cells) and
+the shock-bubble MOOD workload (64^3, 20 % split).
 """
 
 import argparse
@@ -81,6 +87,17 @@ def measured_traffic(kernel, key):
     if not path.exists():
         return None
     m = re.search(rf"traffic_bytes_per_launch {kernel} {key} = (\d+)", path.read_text())
+    return int(m.group(1)) if m else None
+
+
+def measured_traffic_reacting(kernel):
+    """Calibrated HBM bytes per launch of one kernel of the reacting workload
+    from the committed PMC summary (FETCH_SIZE / WRITE_SIZE runs), or None."""
+    import re
+    path = ROOT / "profiles" / "hbm_traffic_reacting_air_86.md"
+    if not path.exists():
+        return None
+    m = re.search(rf"traffic_bytes_per_launch {re.escape(kernel)} = (\d+)", path.read_text())
     return int(m.group(1)) if m else None
 
 
@@ -228,8 +245,87 @@ def mood_rate(n=64, split=0.2, steps=5, seed=1):
     return out
 
 
-def reacting_rate(n=86, split=0.2, steps=5, seed=1, train_cells=1 << 15, train_traj=24,
-                  train_steps=3000):
+def _timing_start(op, mode=2):
+    from hybrid import _native as nat
+    nat.check(nat.lib().fsg_timing(op.device_mesh.ctx, mode))
+
+
+def _timing_read(op):
+    """Summed ms and launch counts per face group and element block since
+    _timing_start (fsg_timing / fsg_timing_read: HIP events on the launch
+    stream around each kernel launch)."""
+    import ctypes
+    from hybrid import _native as nat
+    dm = op.device_mesh
+    gms = (ctypes.c_double * max(len(dm.groups), 1))()
+    bms = (ctypes.c_double * len(dm.block_tables))()
+    gn = (ctypes.c_int64 * max(len(dm.groups), 1))()
+    bn = (ctypes.c_int64 * len(dm.block_tables))()
+    nat.check(nat.lib().fsg_timing_read(dm.ctx, ctypes.addressof(gms), ctypes.addressof(bms),
+                                        ctypes.addressof(gn), ctypes.addressof(bn)))
+    nat.check(nat.lib().fsg_timing(dm.ctx, 0))
+    return list(gms)[:len(dm.groups)], list(gn)[:len(dm.groups)], list(bms), list(bn)
+
+
+TYPE_SHORT = {"hexahedron": "hex", "wedge": "wedge", "tetra": "tet"}
+
+
+def kernel_table(op, p, V, itemsize, timing):
+    """Per timed kernel (face group or element block): name, launches, average
+    ms, and the algorithmic work per launch -- HBM bytes for face kernels
+    (per face: read 2 N_f V B of traces + 4 Q_f B of geometry, write
+    2 N_f V B; one side on boundary faces) and flops for element kernels
+    (the sum-factorised count for hexahedra at p >= 2, the dense-operator
+    count otherwise)."""
+    from hybrid.elements import face_quadrature, reference_tables
+    gms, gn, bms, bn = timing
+    dm = op.device_mesh
+    prec = "f64" if itemsize == 8 else "f32"
+    rows = []
+    for g, ms, n in zip(dm.groups, gms, gn):
+        if not n:
+            continue
+        to = TYPE_SHORT[dm.topo.blocks[g["owner_block"]].element_type]
+        tn = ("boundary" if g["neighbor_block"] < 0
+              else TYPE_SHORT[dm.topo.blocks[g["neighbor_block"]].element_type])
+        ft = ("quad", "tri")[g["face_type"]]
+        qf = len(face_quadrature(("quad", "triangle")[g["face_type"]], p)[1])
+        per_face = 2 * (g["FO"] + g["FN"]) * V * itemsize + 4 * qf * itemsize
+        rows.append(dict(kernel=f"face_kernel<{ft},{to},{tn}{',sub' if g['sub'] else ''},"
+                                f"p{p},V{V},{prec}>", kind="face", launches=int(n),
+                         total_ms=ms, avg_ms=ms / n, count=int(g["count"]),
+                         bytes_per_launch=int(per_face * g["count"])))
+    for b, t, ms, n in zip(dm.topo.blocks, dm.block_tables, bms, bn):
+        if not n:
+            continue
+        R = reference_tables(b.element_type, p)
+        sf = b.element_type == "hexahedron" and p >= 2
+        fl = hex_sf_flops(p, V) if sf else element_flops(R["N"], R["Q"], V)
+        name = (f"hex_sf_kernel<p{p},V{V},{prec}>" if sf else
+                f"element_kernel<{TYPE_SHORT[b.element_type]},p{p},V{V},{prec}>")
+        rows.append(dict(kernel=name, kind="element", launches=int(n), total_ms=ms,
+                         avg_ms=ms / n, count=int(b.count), flops_per_launch=int(fl * b.count)))
+    return sorted(rows, key=lambda r: -r["total_ms"])
+
+
+def roofline_of(row, prec, traffic=None):
+    """The roofline object of one kernel_table row (face: HBM bytes; element:
+    FP32/FP64 flops against the dense matrix-core peak)."""
+    if row["kind"] == "face":
+        gbs = row["bytes_per_launch"] / (row["avg_ms"] * 1e-3) / 1e9
+        return {"kernel": row["kernel"], "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK,
+                "unit": "GB/s", "frac": gbs / HBM_PEAK, "traffic": traffic,
+                "bytes_per_launch": row["bytes_per_launch"], "faces": row["count"],
+                "avg_launch_ms": row["avg_ms"], "launches": row["launches"]}
+    tf = row["flops_per_launch"] / (row["avg_ms"] * 1e-3) / 1e12
+    return {"kernel": row["kernel"], "bound": "mfma", "achieved": tf, "peak": PEAK[prec],
+            "unit": "TFLOP/s", "frac": tf / PEAK[prec], "traffic": traffic,
+            "flops_per_launch": row["flops_per_launch"], "elements": row["count"],
+            "avg_launch_ms": row["avg_ms"], "launches": row["launches"]}
+
+
+def reacting_rate(n=86, split=0.2, steps=20, warmup=2, error_steps=100, seed=1,
+                  train_cells=1 << 15, train_traj=24, train_steps=3000):
     """Reacting hybrid-mesh flow: the shock-bubble mesh family scaled up (unit
     box of n^3 hexahedra, a seeded 20 % split into wedges / tetrahedra; n = 86
     gives about 1.0M elements), p=2, 5 + 11 variables, Mach 3 air at 1500 K
@@ -240,8 +336,10 @@ def reacting_rate(n=86, split=0.2, steps=5, seed=1, train_cells=1 << 15, train_t
     iterations, temperature re-solved inside).  Fast path: FP32 residual and
     the BF16-MFMA surrogate of the same substep with its integrator fallback,
     the surrogate trained here on integrator trajectories started from the
-    flow's own cells.  Same dt for both; a timed step is the RK step and the
-    chemistry."""
+    flow's own cells.  Same dt for both.  A timed step is the RK step and the
+    chemistry; `steps` steps are timed after `warmup` untimed ones, both
+    paths then continue untimed to `error_steps` steps for the L2 error of
+    the fast path against the plain FP64 path."""
     import torch
     from hybrid import (FarfieldWallBoundary, HybridCompressibleSolver, HybridMOODSubcellLimiter,
                         HybridNavierStokesOperator, HybridPositivityLimiter, HybridSSPRK3,
@@ -263,6 +361,7 @@ def reacting_rate(n=86, split=0.2, steps=5, seed=1, train_cells=1 << 15, train_t
                        "MOOD + positivity, Park-11 chemistry after every step",
            "mesh_s": time.perf_counter() - t0}
     res, dt = {}, None
+    error_steps = max(error_steps, steps)
     for key, dtype, use_sur in (("f64_plain", torch.float64, False),
                                 ("f32_surrogate", torch.float32, True)):
         t0 = time.perf_counter()
@@ -307,66 +406,101 @@ def reacting_rate(n=86, split=0.2, steps=5, seed=1, train_cells=1 << 15, train_t
                                             loss_first_last=[hist[0], hist[-1]],
                                             seconds=time.perf_counter() - ts)
             del X, O
-        U, valid, _, _ = rk.checked_step(U0, dt, bnd)        # warm-up (builds the MOOD tables)
-        U = coupler.advance(U, dt)
+
+        fallback = torch.zeros((), dtype=torch.int64, device="cuda")
+        chem_ev = []
+        rejected = [0]
+
+        def step(U, record=False):
+            cand, valid, _, _ = rk.checked_step(U, dt, bnd)
+            if not bool(valid):                  # the reference's one host sync per step
+                rejected[0] += 1
+                return U
+            a = torch.cuda.Event(enable_timing=True) if record else None
+            b = torch.cuda.Event(enable_timing=True) if record else None
+            if record:
+                a.record()
+            nxt = {}
+            for t, v in cand.items():
+                nxt[t] = coupler.react_block(v, dt)
+                if use_sur and coupler.last_fallback is not None:
+                    fallback.add_(coupler.last_fallback.sum())
+            if record:
+                b.record()
+                chem_ev.append((a, b))
+            return nxt
+
+        U = U0
+        for _ in range(warmup):                  # builds the MOOD tables, warms the caches
+            U = step(U)
         torch.cuda.synchronize()
         entry["setup_s"] = time.perf_counter() - t0
-        chem_ev = []
-        U = U0
-        torch.cuda.synchronize()
+        fallback.zero_()
+        rejected[0] = 0
+        _timing_start(op, 2)
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
+        U = U0
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
         ev0.record()
-        rejected = fallback = 0
         for _ in range(steps):
-            cand, valid, rmin, pmin = rk.checked_step(U, dt, bnd)
-            if not bool(valid):
-                rejected += 1
-                continue
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            U = coupler.advance(cand, dt)
-            b.record()
-            chem_ev.append((a, b))
-            if use_sur:
-                fallback += int(coupler.last_fallback.sum())
+            U = step(U, record=True)
         ev1.record()
         torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
         ms = ev0.elapsed_time(ev1)
+        timing = _timing_read(op)
         chem_ms = sum(a.elapsed_time(b) for a, b in chem_ev)
         dof = sum(v.shape[0] * v.shape[1] for v in U.values())
         E = sum(b.count for b in solver.mesh.blocks)
-        entry.update(ms_per_step=ms / steps, chemistry_ms_per_step=chem_ms / steps,
+        itemsize = 8 if dtype == torch.float64 else 4
+        ktab = kernel_table(op, 2, 16, itemsize, timing)
+        entry.update(ms_per_step=ms / steps, wall_ms_per_step=wall * 1e3 / steps,
+                     chemistry_ms_per_step=chem_ms / steps,
                      dof_updates_per_s_per_stage=dof * 3 * steps / (ms * 1e-3),
-                     rejected_steps=rejected, troubled_last_stage=lim.troubled_count())
+                     rejected_steps=rejected[0], troubled_last_stage=lim.troubled_count(),
+                     kernels=ktab[:6])
         if use_sur:
-            entry["fallback_fraction"] = fallback / max(dof * (steps - rejected), 1)
-        res[key] = (entry, U)
+            entry["fallback_fraction"] = int(fallback) / max(dof * (steps - rejected[0]), 1)
+        at_timed = {k: v.clone() for k, v in U.items()}
+        for _ in range(error_steps - steps):
+            U = step(U)
+        torch.cuda.synchronize()
+        res[key] = (entry, at_timed, U)
         out["elements"] = {b.element_type: b.count for b in solver.mesh.blocks}
         out["elements_total"], out["dof_per_var"] = E, dof
         print(f"[bench] reacting {key}: {ms / steps:.2f} ms/step (chemistry "
-              f"{chem_ms / steps:.2f}), {E} elements, troubled {entry['troubled_last_stage']}",
-              file=sys.stderr, flush=True)
+              f"{chem_ms / steps:.2f}), {E} elements, troubled {entry['troubled_last_stage']}, "
+              f"top kernel {ktab[0]['kernel']} {ktab[0]['avg_ms']:.3f} ms", file=sys.stderr,
+              flush=True)
         del rk, lim, pos, op, solver, coupler
         torch.cuda.empty_cache()
     for key in res:
         out[key] = res[key][0]
-    a, b = res["f32_surrogate"][1], res["f64_plain"][1]
-    A = torch.cat([a[k].double().reshape(-1, 16) for k in b])
-    B = torch.cat([b[k].reshape(-1, 16) for k in b])
-    diff = torch.linalg.vector_norm(A - B, dim=0)
-    norm = torch.linalg.vector_norm(B, dim=0)
     labels = ["rho", "rho_u", "rho_v", "rho_w", "rho_E"] + [f"rho_{s}" for s in SPECIES_NAMES]
-    # relative L2 per variable for the variables with O(1) norms; v, w and the
-    # species are measured against the density norm (their own norms are
-    # round-off or tiny in this stream)
-    out["fast_rel_l2_vs_plain_fp64"] = {labels[i]: float(diff[i] / norm[i]) for i in (0, 1, 4)}
-    out["fast_l2_over_rho_l2"] = {k: float(v) for k, v in zip(labels, (diff / norm[0]).cpu())}
-    out["species_rel_l2"] = {f"rho_{s}": float(diff[5 + i] / norm[5 + i])
-                             for i, s in enumerate(SPECIES_NAMES) if s in ("N2", "O2", "O", "NO")}
+
+    def errors(a, b):
+        A = torch.cat([a[k].double().reshape(-1, 16) for k in b])
+        B = torch.cat([b[k].reshape(-1, 16) for k in b])
+        diff = torch.linalg.vector_norm(A - B, dim=0)
+        norm = torch.linalg.vector_norm(B, dim=0)
+        # relative L2 per variable for the flow variables with O(1) norms and
+        # the species; v and w are measured against the rho u norm (their
+        # own norms are the 1 % perturbation's)
+        d = {labels[i]: float(diff[i] / norm[i]) for i in (0, 1, 4)}
+        d["rho_v"], d["rho_w"] = float(diff[2] / norm[1]), float(diff[3] / norm[1])
+        sp = {f"rho_{s}": float(diff[5 + i] / norm[5 + i])
+              for i, s in enumerate(SPECIES_NAMES) if s in ("N2", "O2", "O", "NO", "N")}
+        return d, sp
+
+    out["fast_rel_l2_vs_plain_fp64"], out["species_rel_l2"] = errors(
+        res["f32_surrogate"][1], res["f64_plain"][1])
+    out["fast_rel_l2_vs_plain_fp64_at_error_steps"], out["species_rel_l2_at_error_steps"] = \
+        errors(res["f32_surrogate"][2], res["f64_plain"][2])
     out["speedup_fast_vs_plain"] = (out["f64_plain"]["ms_per_step"]
                                     / out["f32_surrogate"]["ms_per_step"])
-    out["steps"], out["dt"] = steps, dt
+    out["steps"], out["warmup"], out["error_steps"], out["dt"] = steps, warmup, error_steps, dt
     return out
 
 
@@ -456,7 +590,8 @@ def chemistry_rate(cells, seed=11, train_cells=1 << 18, train_steps=3000):
 
 
 def cpu_baseline():
-    """CPU FP64 oracle, one thread, on 8^3 elements for 2 RK stages."""
+    """CPU FP64 oracle, one thread, on the Taylor-Green vortex (8^3 elements,
+    p=3) for 2 RK stages."""
     code = r"""
 import sys, time, json
 sys.path.insert(0, %r); sys.path.insert(0, %r)
@@ -491,24 +626,165 @@ print(json.dumps({"dof": 512 * 64, "stage_s": t}))
                       "2 SSP-RK3 stages, 1 thread; per-DOF rate"}
 
 
+REACTING_CPU_SAMPLE = r"""
+import sys, time, json
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import numpy as np
+from bench import reacting_inflow_primitive, GAMMA, PR
+from hybrid.meshgen import box_mesh
+from oracle.mesh import from_arrays
+from oracle.solver import CompressibleSolver, FarfieldWallBoundary
+from oracle.operators import NavierStokesOperator
+from oracle.limiter import MOODSubcellLimiter, PositivityLimiter
+from oracle.time_integration import SSPRK3
+from oracle.chemistry import ReactingFlowCoupler
+names = {"xmin": "inlet", "xmax": "outlet", "ymin": "walls", "ymax": "walls",
+         "zmin": "walls", "zmax": "walls"}
+m = box_mesh(3, split_fraction=0.2, seed=1, plane_names=names)
+_, far = reacting_inflow_primitive(np.zeros((1, 3)))
+s = CompressibleSolver(gamma=GAMMA, dynamic_viscosity=far[1] / 1e4, prandtl=PR, num_species=11)
+op = NavierStokesOperator(from_arrays(m), s, 2)
+lim = MOODSubcellLimiter(op, positivity=PositivityLimiter(op, density_floor=1e-7,
+                                                          pressure_floor=1e-7))
+pn = m.physical_names
+ob = FarfieldWallBoundary(s, far, inlet_tags=[pn["inlet"]], outlet_tags=[pn["outlet"]],
+                          wall_tags=[pn["walls"]], slip_tags=[pn["walls"]])
+st = {t: s.primitive_to_conserved(reacting_inflow_primitive(x)[0])
+      for t, x in op.geometry.solution_node_coordinates.items()}
+rk = SSPRK3(op, cfl=0.3, limiter=lim)
+dt = rk.stable_dt(st)
+cp = ReactingFlowCoupler(max_newton_iters=12, dimensionless=True, gamma=GAMMA)
+t0 = time.perf_counter()
+u, ok, _, _ = rk.checked_step(st, dt, ob)
+t1 = time.perf_counter()
+flat = np.concatenate([v.reshape(-1, v.shape[-1]) for v in u.values()])
+nodes = %d
+cp.react_block(flat[None, :nodes], dt)
+t2 = time.perf_counter()
+print(json.dumps({"dof": int(flat.shape[0]), "flow_s": t1 - t0, "chem_s": t2 - t1,
+                  "chem_nodes": nodes}))
+"""
+
+
+def cpu_baseline_reacting(nodes=160):
+    """The CPU FP64 oracle on a bounded sample of the reacting workload: one
+    SSP-RK3 step with MOOD + positivity on a 3^3 box with the same 20 % split
+    (22 hexahedra + 30 tetrahedra, 894 nodes, 16 variables), and the Park-11
+    coupler on `nodes` of its nodes, its time extrapolated per node to all
+    of them.  Run once alone on one thread, then as one single-thread copy
+    per available core at once (all-core throughput: the sum of the copies'
+    rates, each measured while all run)."""
+    code = REACTING_CPU_SAMPLE % (str(ROOT), str(ROOT / "jax-fluid-sim_amd"), nodes)
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+
+    def launch(k):
+        return [subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE,
+                                 stderr=subprocess.PIPE, text=True, env=env, cwd=str(ROOT))
+                for _ in range(k)]
+
+    def rates(procs):
+        out = []
+        for pr in procs:
+            so, se = pr.communicate(timeout=900)
+            if pr.returncode != 0:
+                raise RuntimeError(se[-400:])
+            d = json.loads(so.strip().splitlines()[-1])
+            t = d["flow_s"] + d["chem_s"] * d["dof"] / d["chem_nodes"]
+            out.append((d["dof"] * 3 / t, d))
+        return out
+
+    cores = len(os.sched_getaffinity(0))
+    try:
+        one = rates(launch(1))[0]
+        allc = rates(launch(cores))
+    except Exception as e:                              # noqa: BLE001
+        return {"value": None, "error": str(e)}
+    d = one[1]
+    print(f"[bench] CPU oracle reacting sample: {one[0]:.1f} DOF/s/stage on 1 thread, "
+          f"{sum(r for r, _ in allc):.1f} on {cores} cores", file=sys.stderr, flush=True)
+    return {"value": sum(r for r, _ in allc), "unit": "DOF-updates/s per RK stage",
+            "cores": cores, "kind": "port", "one_thread_value": one[0],
+            "sample": f"CPU FP64 oracle (numpy) on the reacting workload down-scaled: 3^3 box, "
+                      f"20 % split (22 hex + 30 tet, {d['dof']} nodes x 16 variables), one "
+                      f"SSP-RK3 step with MOOD + positivity ({d['flow_s']:.2f} s on one thread) "
+                      f"and the Park-11 coupler on {nodes} nodes ({d['chem_s']:.2f} s), "
+                      "extrapolated per node; value = sum of one single-thread copy per core "
+                      "running at once; one_thread_value = one copy alone"}
+
+
+def tgv_rate(n, steps, warmup, element_path="auto"):
+    """Taylor-Green vortex, periodic 32^3 hexahedra, p=3 (the plain FP64 and
+    the FP32 residual paths with the same dt), with the face and element
+    kernels' own launch times."""
+    import torch
+    from hybrid import Topology, box_mesh
+    topo = Topology(box_mesh(n, lengths=(2 * np.pi,) * 3, periodic=(True,) * 3))
+    E = topo.blocks[0].count
+    N, Q, V = 64, 125, 5
+    dof = E * N
+    print(f"[bench] TGV {n}^3 p=3: {E} elements, {dof} DOF/var", file=sys.stderr, flush=True)
+    res, dt = {}, None
+    for name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        r = run_path(topo, dtype, steps, warmup, dt, element_path)
+        dt = r["dt"]
+        r["rate"] = dof * 3 * steps / r["elapsed"]
+        res[name] = r
+        print(f"[bench] TGV {name}: {r['elapsed'] * 1e3 / steps:.3f} ms/step, "
+              f"{r['rate']:.4e} DOF/s/stage, element {r['elem_ms']:.3f} ms, "
+              f"face {r['face_ms']:.3f} ms, valid={r['valid']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    a = res["f32"]["state"]["hexahedron"].double()
+    b = res["f64"]["state"]["hexahedron"]
+    err = (torch.linalg.vector_norm((a - b).reshape(-1, V), dim=0)
+           / torch.linalg.vector_norm(b.reshape(-1, V), dim=0)).cpu().numpy()
+    out = {"workload": f"Taylor-Green vortex, periodic {n}^3 hexahedra, DG p=3, HLLC + BR2, "
+                       "SSP-RK3, CFL 0.3, Mach 0.1, Re 1600, Pr 0.71",
+           "elements": E, "dof_per_var": dof, "dt": dt, "steps": steps,
+           "dt_clipped": bool(dt >= 1e-3 or dt <= 1e-7)}
+    nfaces = 3 * E
+    for key in ("f64", "f32"):
+        r = res[key]
+        itemsize = 4 if key == "f32" else 8
+        fb = (2 * 16 * V * 2 + 4 * 16) * itemsize * nfaces          # per face: traces + geometry
+        eflops = hex_sf_flops(3, V) * E
+        out[key] = {
+            "ms_per_step": r["elapsed"] * 1e3 / steps, "dof_updates_per_s_per_stage": r["rate"],
+            "face_kernel": {"kernel": f"face_kernel<quad,hex,hex,p3,V5,{key}>",
+                            "avg_launch_ms": r["face_ms"], "bytes_per_launch": fb,
+                            "achieved_GBs": fb / (r["face_ms"] * 1e-3) / 1e9,
+                            "frac_hbm": fb / (r["face_ms"] * 1e-3) / 1e9 / HBM_PEAK,
+                            "traffic": measured_traffic("face", key) if n == 32 else None},
+            "element_kernel": {"kernel": f"hex_sf_kernel<p3,V5,{key}> (sum factorisation)",
+                               "avg_launch_ms": r["elem_ms"], "flops_per_launch": eflops,
+                               "achieved_tflops": eflops / (r["elem_ms"] * 1e-3) / 1e12,
+                               "frac": eflops / (r["elem_ms"] * 1e-3) / 1e12 / PEAK[key],
+                               "traffic": measured_traffic("element", key) if n == 32 else None,
+                               "dense_operator_flops_per_launch": element_flops(N, Q, V) * E}}
+    out["fp32_rel_l2_vs_fp64"] = dict(zip(["rho", "rho_u", "rho_v", "rho_w", "rho_E"],
+                                          map(float, err)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline workload")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=32, help="elements per box edge")
+    ap.add_argument("--error-steps", type=int, default=100,
+                    help="steps after which the fast path's L2 error is measured (untimed "
+                         "beyond --steps)")
+    ap.add_argument("--reacting-n", type=int, default=86)
+    ap.add_argument("--tgv-n", type=int, default=32)
+    ap.add_argument("--tgv-steps", type=int, default=20)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-chem", action="store_true")
     ap.add_argument("--skip-mood", action="store_true")
-    ap.add_argument("--only", choices=["f64", "f32", "mood", "chem", "reacting"], default=None,
-                    help="run one path of the residual benchmark, or only the MOOD or "
-                         "chemistry section (for profiling; prints that section's JSON)")
+    ap.add_argument("--skip-tgv", action="store_true")
+    ap.add_argument("--only", choices=["tgv", "mood", "chem", "reacting"], default=None,
+                    help="run one section only and print its JSON (for profiling)")
     ap.add_argument("--mood-steps", type=int, default=5)
-    ap.add_argument("--skip-reacting", action="store_true")
-    ap.add_argument("--reacting-steps", type=int, default=5)
-    ap.add_argument("--reacting-n", type=int, default=86)
     ap.add_argument("--element-path", choices=["auto", "mfma", "vector"], default="auto",
-                    help="element operators on the matrix cores or the vector units "
+                    help="TGV element operators on the matrix cores or the vector units "
                          "(auto: the library default, vector units)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -518,84 +794,52 @@ def main():
     if args.only == "mood":
         print(json.dumps(mood_rate(steps=args.mood_steps)))
         return
-    if args.only == "reacting":
-        print(json.dumps(reacting_rate(n=args.reacting_n, steps=args.reacting_steps)))
+    if args.only == "tgv":
+        print(json.dumps(tgv_rate(args.tgv_n, args.tgv_steps, args.warmup, args.element_path)))
         return
     if args.only == "chem":
         print(json.dumps(chemistry_rate(1 << 20)))
         return
-
-    import torch
-    from hybrid import Topology, box_mesh
-    topo = Topology(box_mesh(args.n, lengths=(2 * np.pi,) * 3, periodic=(True,) * 3))
-    E = topo.blocks[0].count
-    N, Q, V = 64, 125, 5
-    dof = E * N
-    print(f"[bench] TGV {args.n}^3 p=3: {E} elements, {dof} DOF/var", file=sys.stderr, flush=True)
-
-    res = {}
-    dt = None
-    for name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
-        if args.only and args.only != name:
-            continue
-        r = run_path(topo, dtype, args.steps, args.warmup, dt, args.element_path)
-        dt = r["dt"]
-        r["rate"] = dof * 3 * args.steps / r["elapsed"]
-        res[name] = r
-        print(f"[bench] {name}: {r['elapsed'] * 1e3 / args.steps:.3f} ms/step, "
-              f"{r['rate']:.4e} DOF/s/stage, element {r['elem_ms']:.3f} ms, "
-              f"face {r['face_ms']:.3f} ms, valid={r['valid']}", file=sys.stderr, flush=True)
-        torch.cuda.empty_cache()
-
-    err = None
-    if "f64" in res and "f32" in res:
-        a = res["f32"]["state"]["hexahedron"].double()
-        b = res["f64"]["state"]["hexahedron"]
-        err = (torch.linalg.vector_norm((a - b).reshape(-1, V), dim=0)
-               / torch.linalg.vector_norm(b.reshape(-1, V), dim=0)).cpu().numpy()
-    within = err is not None and err[0] <= 1e-4 and err[4] <= 1e-4
-    key = "f32" if within else ("f64" if "f64" in res else "f32")
-    r = res[key]
-    nfaces = 3 * E
-    itemsize = 4 if key == "f32" else 8
-    fbytes = face_bytes(N, 16, V, itemsize) * nfaces
-    face_gbs = fbytes / (r["face_ms"] * 1e-3) / 1e9
-    eflops = hex_sf_flops(3, V) * E
-    elem_tf = eflops / (r["elem_ms"] * 1e-3) / 1e12
-    traffic = measured_traffic("face", key) if args.n == 32 else None
+    r = reacting_rate(n=args.reacting_n, steps=args.steps, warmup=args.warmup,
+                      error_steps=args.error_steps)
+    if args.only == "reacting":
+        print(json.dumps(r))
+        return
+    fast, plain = r["f32_surrogate"], r["f64_plain"]
+    top = fast["kernels"][0]
+    traffic = measured_traffic_reacting(top["kernel"])
+    err = r["fast_rel_l2_vs_plain_fp64_at_error_steps"]
     out = {
         "metric": "DOF-updates/s per RK stage, plain FP64 vs FP32+surrogate chem, at fixed L2 error",
-        "value": r["rate"], "unit": "DOF-updates/s per RK stage", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": r["elapsed"] * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": key, "data": "synthetic",
-        "config": {"workload": f"Taylor-Green vortex, periodic {args.n}^3 hexahedra, DG p=3, "
-                               "HLLC + BR2, SSP-RK3, CFL 0.3, Mach 0.1, Re 1600, Pr 0.71",
-                   "elements": E, "dof_per_var": dof, "dt": dt,
-                   "dt_clipped": bool(dt >= 1e-3 or dt <= 1e-7)},
-        "fp64_dof_updates_per_s": res.get("f64", {}).get("rate"),
-        "fp32_dof_updates_per_s": res.get("f32", {}).get("rate"),
-        "fp32_rel_l2_vs_fp64": None if err is None else
-            dict(zip(["rho", "rho_u", "rho_v", "rho_w", "rho_E"], map(float, err))),
-        "roofline": {"kernel": f"face_kernel<quad,hex,hex,p3,V5,{key}>", "bound": "hbm",
-                     "achieved": face_gbs, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": face_gbs / HBM_PEAK, "traffic": traffic,
-                     "bytes_per_launch": fbytes, "faces": nfaces, "avg_launch_ms": r["face_ms"]},
-        "element_kernel": {"kernel": f"hex_sf_kernel<p3,V5,{key}> (sum factorisation)",
-                           "avg_launch_ms": r["elem_ms"], "flops_per_launch": eflops,
-                           "achieved_tflops": elem_tf, "peak_tflops": PEAK[key],
-                           "frac": elem_tf / PEAK[key],
-                           "traffic": measured_traffic("element", key) if args.n == 32 else None,
-                           "dense_operator_flops_per_launch": element_flops(N, Q, V) * E},
+        "value": fast["dof_updates_per_s_per_stage"], "unit": "DOF-updates/s per RK stage",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": fast["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": r["workload"], "elements": r["elements"],
+                   "elements_total": r["elements_total"], "dof_per_var": r["dof_per_var"],
+                   "dt": r["dt"], "chemistry": "BF16-MFMA surrogate of the 12-iteration "
+                   "point-implicit substep (FP64 projection, FP64 integrator fallback)"},
+        "plain_fp64_dof_updates_per_s": plain["dof_updates_per_s_per_stage"],
+        "plain_fp64_ms_per_step": plain["ms_per_step"],
+        "speedup_fast_vs_plain": r["speedup_fast_vs_plain"],
+        "rel_l2_vs_plain_fp64": {"after_steps": r["error_steps"], **err,
+                                 "species": r["species_rel_l2_at_error_steps"]},
+        "rel_l2_vs_plain_fp64_after_timed_steps": {**r["fast_rel_l2_vs_plain_fp64"],
+                                                   "species": r["species_rel_l2"]},
+        "within_1e-4": bool(err["rho"] <= 1e-4 and err["rho_E"] <= 1e-4),
+        "roofline": roofline_of(top, "f32", traffic),
+        "reacting": r,
     }
+    if not args.skip_tgv:
+        out["tgv"] = tgv_rate(args.tgv_n, args.tgv_steps, args.warmup, args.element_path)
     if not args.skip_chem:
         out["chemistry"] = chemistry_rate(1 << 20)
     if not args.skip_mood:
         out["mood"] = mood_rate(steps=args.mood_steps)
-    if not args.skip_reacting:
-        out["reacting"] = reacting_rate(n=args.reacting_n, steps=args.reacting_steps)
     if not args.skip_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline_reacting()
+        if "tgv" in out:
+            out["tgv"]["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
 
 

@@ -180,7 +180,9 @@ int fsg_diag_reset(fsg_stage_diag* diag, void* stream);
 /* Per-launch timing with HIP events recorded on the launch stream around each
  * face-group and element-block kernel (no counterpart in the reference; used
  * by the benchmark to report a kernel's own duration).  fsg_timing(ctx, 1)
- * starts a fresh recording; fsg_timing_read waits for the recorded events,
+ * starts a fresh recording of every launch, fsg_timing(ctx, 2) of the
+ * launches of full (unmasked) residual passes only, fsg_timing(ctx, 0) stops
+ * recording; fsg_timing_read waits for the recorded events,
  * writes the summed milliseconds and launch counts per group / block (arrays
  * of ngroups / nblocks entries, any may be null) and clears the record. */
 int fsg_timing(fsg_context* ctx, int32_t enable);
@@ -208,6 +210,83 @@ const char* fsg_last_error(void);
  * of the operand / accumulator lane layout). */
 int fsg_mfma_selftest(int32_t dtype, const void* A, const void* B, void* C, void* stream);
 int32_t fsg_abi_version(void);
+
+/* ---- Time-step control ------------------------------------------------------
+ * The CFL estimate HybridSSPRK3.stable_dt (hybrid/time_integration.py:93-169,
+ * without WALE), the element-local dt local_dt (:171-262) and the
+ * accept / reject loop of run_hybrid_sphere.py:265-346 on the device, plus
+ * the FP64 conservation sums (sum over elements of int U dV).  A dt context
+ * extends a residual context (same blocks). */
+typedef struct fsg_dt fsg_dt;
+
+/* Per element block (device arrays, FP64): */
+typedef struct {
+  const double* length;       /* (E) max(2V / sum of face areas, V^(1/3) / (4 (p+1))) */
+  const double* sliver;       /* (E) min(1, V / 1e-4) */
+  const double* detw;         /* (E) |det J| */
+  const double* node_weights; /* (N) integral of each basis function over the reference element */
+} fsg_dt_block;
+
+/* One group of interior faces of the local-dt smoothing, in the order of
+ * hybrid/time_integration.py:27-47 (face type, owner block, neighbour
+ * block). */
+typedef struct {
+  int32_t owner_block;
+  int32_t neighbor_block;
+  int64_t count;
+  const int32_t* owner_cells;
+  const int32_t* neighbor_cells;
+} fsg_dt_conn;
+
+/* Step-loop state in device memory (the caller initialises it): dt =
+ * min(retry * cfl_dt, target_time - time); an accepted step advances time
+ * and step and grows retry by 1.25 up to 1; a rejected one halves retry;
+ * maximum_rejections consecutive rejections set status = 1, after which
+ * (as after target_time or maximum_steps) steps do nothing.  cfl_dt is
+ * refreshed by fsg_stable_dt when step % refresh_interval == 0. */
+typedef struct {
+  double cfl_dt, retry, time, target_time, dt, factor, min_dt, last_rho_min, last_p_min;
+  int64_t step, maximum_steps, attempts, rejections;
+  int32_t consecutive, maximum_rejections, status, active, accept, have_dt, refresh_interval,
+      reserved;
+} fsg_step_state;
+
+/* work: device FP64 scratch of fsg_dt_work_size() entries, owned by the caller. */
+int64_t fsg_dt_work_size(const fsg_context* ctx, int32_t nconn, const fsg_dt_conn* conns);
+int fsg_dt_create(fsg_context* ctx, const fsg_dt_block* blocks, int32_t nconn,
+                  const fsg_dt_conn* conns, double* work, int64_t work_len, fsg_dt** out);
+void fsg_dt_destroy(fsg_dt* d);
+
+/* HybridSSPRK3.stable_dt (hybrid/time_integration.py:93-169): the clipped
+ * CFL dt into dt_out (device double, may be null), and into state->cfl_dt
+ * when state is given and its refresh is due. */
+int fsg_stable_dt(fsg_dt* d, const void* const* U, double cfl, double* dt_out,
+                  fsg_step_state* state, void* stream);
+
+/* HybridSSPRK3.local_dt (hybrid/time_integration.py:171-262): per-element
+ * dt into dt_elem[b] (context dtype, (E_b)) and dt_elem64[b] (FP64, may be
+ * null), multiplied by *factor (device double, may be null). */
+int fsg_local_dt(fsg_dt* d, const void* const* U, double cfl, void* const* dt_elem,
+                 double* const* dt_elem64, const double* factor, void* stream);
+
+/* sum over all elements of int_K U dV per variable, in FP64 (out: nvars
+ * device doubles), reduced in a fixed order (deterministic). */
+int fsg_conserved_totals(fsg_dt* d, const void* const* U, double* out, void* stream);
+
+/* Step loop on the device (run_hybrid_sphere.py:265-346): begin fills the
+ * per-element dt of the attempt (context dtype, and FP64 for the chemistry)
+ * from the state, or the retry-scaled local dt when lts != 0; the caller
+ * then runs the stages (fsg_rk_stage / fsg_mood_stage_fused with those dt
+ * arrays) and the chemistry on the candidate; commit reads the stage
+ * diagnostics, updates the state and copies the candidate into states when
+ * the step is accepted. */
+int fsg_step_begin(fsg_dt* d, fsg_step_state* state, void* const* dt_elem,
+                   double* const* dt_elem64, int32_t lts, const void* const* U, double cfl,
+                   void* stream);
+int fsg_step_commit(fsg_dt* d, fsg_step_state* state, const fsg_stage_diag* diag,
+                    double density_floor, double pressure_floor, const void* const* candidate,
+                    void* const* states, void* stream);
+const char* fsg_dt_last_error(void);
 
 /* ---- MOOD: troubled-cell detection, subcell FV fallback, positivity --------
  * HybridMOODSubcellLimiter (hybrid/limiter.py:267-343) with its

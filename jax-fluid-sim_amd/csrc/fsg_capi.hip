@@ -46,8 +46,8 @@ int fsg_fail(int code, const char* fmt, ...) {
 
 namespace {
 // Records the start event of a timed launch; returns the pair index or -1.
-long timing_begin(fsg_context* c, int kind, hipStream_t s) {
-  if (!c->timing) return -1;
+long timing_begin(fsg_context* c, int kind, hipStream_t s, bool masked) {
+  if (!c->timing || (c->timing == 2 && masked)) return -1;
   size_t i = c->used;
   if (2 * (i + 1) > c->pool.size()) {
     hipEvent_t a, b;
@@ -132,7 +132,7 @@ int run_faces(fsg_context* c, const void* const* U, int terms, const int32_t* co
     }
     a.count = g.count;
     a.terms = terms;
-    long ev = timing_begin(c, int(&g - c->groups.data()), s);
+    long ev = timing_begin(c, int(&g - c->groups.data()), s, mask != nullptr);
     int rc = launch_face<P, V, real>(g.face_type, bo.element_type, tn, g.sub, g.count, a, ph, s);
     timing_end(c, ev, s);
     if (rc) return fail(rc, "face launch failed (type %d/%d/%d sub %d): %s", g.face_type,
@@ -177,7 +177,7 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.terms = terms;
     a.diag = reinterpret_cast<StageDiag*>(diag);
     a.mask = mask ? mask[b] : nullptr;
-    long ev = timing_begin(c, int(c->groups.size() + b), s);
+    long ev = timing_begin(c, int(c->groups.size() + b), s, mask != nullptr);
     const int mfma = c->element_mfma > 0 ? 1 : 0;
     int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s, mfma);
     timing_end(c, ev, s);
@@ -304,7 +304,7 @@ int fsg_set_option(fsg_context* ctx, int32_t option, int32_t value) {
 
 int fsg_timing(fsg_context* ctx, int32_t enable) {
   if (!ctx) return fail(-10, "null argument");
-  ctx->timing = enable != 0;
+  ctx->timing = enable == 2 ? 2 : (enable != 0 ? 1 : 0);
   ctx->used = 0;
   return 0;
 }

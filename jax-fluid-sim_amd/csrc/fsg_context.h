@@ -14,13 +14,14 @@ struct fsg_context {
   const int32_t* support;
   void* face_out;
   // Optional per-launch timing: an event pair around every launch, on the
-  // launch stream; kind = group index, or ngroups + block index.
-  bool timing = false;
+  // launch stream; kind = group index, or ngroups + block index.  1 times
+  // every launch, 2 only the launches of unmasked residual passes.
+  int timing = 0;
   std::vector<hipEvent_t> pool;
   std::vector<int> kinds;
   size_t used = 0;
-  int element_mfma = -1;
-  int hex_sum_fact = -1;  // FSG_OPT_HEX_SUM_FACTORISATION; -1: default (on)  // FSG_OPT_ELEMENT_MFMA; -1: default (vector units)
+  int element_mfma = -1;   // FSG_OPT_ELEMENT_MFMA; -1: default (vector units)
+  int hex_sum_fact = -1;  // FSG_OPT_HEX_SUM_FACTORISATION; -1: default (on)
 };
 
 // Records a formatted error for fsg_last_error() and returns code.

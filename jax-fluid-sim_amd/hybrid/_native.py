@@ -76,6 +76,26 @@ class ChemParams(ctypes.Structure):
                 ("t_ref", ctypes.c_double)]
 
 
+class DtBlock(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("length", "sliver", "detw", "node_weights")]
+
+
+class DtConn(ctypes.Structure):
+    _fields_ = [("owner_block", ctypes.c_int32), ("neighbor_block", ctypes.c_int32),
+                ("count", ctypes.c_int64), ("owner_cells", ctypes.c_void_p),
+                ("neighbor_cells", ctypes.c_void_p)]
+
+
+class StepState(ctypes.Structure):
+    """fsg_step_state of include/fluid_gfx950.h (held in device memory)."""
+    _fields_ = [(k, ctypes.c_double) for k in ("cfl_dt", "retry", "time", "target_time", "dt",
+                                               "factor", "min_dt", "last_rho_min",
+                                               "last_p_min")] + [
+        (k, ctypes.c_int64) for k in ("step", "maximum_steps", "attempts", "rejections")] + [
+        (k, ctypes.c_int32) for k in ("consecutive", "maximum_rejections", "status", "active",
+                                      "accept", "have_dt", "refresh_interval", "reserved")]
+
+
 class Boundary(ctypes.Structure):
     _fields_ = [("has_boundary", ctypes.c_int32), ("has_wall_temperature", ctypes.c_int32),
                 ("wall_temperature", ctypes.c_double), ("farfield", ctypes.c_double * 16)]
@@ -153,6 +173,22 @@ def lib():
                      "fsg_mood_recompute", "fsg_mood_stage", "fsg_mood_stage_fused",
                      "fsg_mood_count"):
             getattr(L, name).restype = ctypes.c_int
+        pp = ctypes.POINTER(vp)
+        L.fsg_dt_work_size.argtypes = [vp, i32, ctypes.POINTER(DtConn)]
+        L.fsg_dt_work_size.restype = ctypes.c_int64
+        L.fsg_dt_create.argtypes = [vp, ctypes.POINTER(DtBlock), i32, ctypes.POINTER(DtConn), vp,
+                                    i64, ctypes.POINTER(vp)]
+        L.fsg_dt_destroy.argtypes = [vp]
+        L.fsg_dt_destroy.restype = None
+        L.fsg_stable_dt.argtypes = [vp, pp, f64, vp, vp, vp]
+        L.fsg_local_dt.argtypes = [vp, pp, f64, pp, pp, vp, vp]
+        L.fsg_conserved_totals.argtypes = [vp, pp, vp, vp]
+        L.fsg_step_begin.argtypes = [vp, vp, pp, pp, i32, pp, f64, vp]
+        L.fsg_step_commit.argtypes = [vp, vp, vp, f64, f64, pp, pp, vp]
+        for name in ("fsg_dt_create", "fsg_stable_dt", "fsg_local_dt", "fsg_conserved_totals",
+                     "fsg_step_begin", "fsg_step_commit"):
+            getattr(L, name).restype = ctypes.c_int
+        L.fsg_dt_last_error.restype = ctypes.c_char_p
         f32 = ctypes.c_float
         L.fsg_mlp_linear_forward.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]
         L.fsg_mlp_linear_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32,
@@ -175,6 +211,12 @@ def lib():
 def check(rc):
     if rc != 0:
         raise RuntimeError(f"libfluid_gfx950 error {rc}: {lib().fsg_last_error().decode()}")
+
+
+def check_dt(rc):
+    if rc != 0:
+        raise RuntimeError(f"libfluid_gfx950 time-step error {rc}: "
+                           f"{lib().fsg_dt_last_error().decode()}")
 
 
 def check_chem(rc):

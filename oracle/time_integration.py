@@ -61,6 +61,126 @@ class SSPRK3:
             rates.append(rate.max())
         return float(np.clip(self.cfl / max(max(rates), 1e-12), 1e-7, 1e-3))
 
+    def internal_connections(self):
+        """hybrid/time_integration.py:27-47: (owner type, owner cells, neighbour
+        type, neighbour cells) per face type (mesh order) and owner x
+        neighbour block pair (ascending)."""
+        mesh = self.operator.mesh
+        out = []
+        for ftype, fb in mesh.face_blocks.items():
+            inner = np.flatnonzero(fb.neighbor_block >= 0)
+            if not len(inner):
+                continue
+            ob, nb = fb.owner_block[inner], fb.neighbor_block[inner]
+            for o in np.unique(ob):
+                for n in np.unique(nb):
+                    sel = inner[(ob == o) & (nb == n)]
+                    if len(sel):
+                        out.append((mesh.element_blocks[o].element_type, fb.owner_local[sel],
+                                    mesh.element_blocks[n].element_type, fb.neighbor_local[sel]))
+        return out
+
+    def _element_rates(self, states):
+        """Per-element stability rate of hybrid/time_integration.py:93-169 (no
+        WALE), before the NaN replacement."""
+        s = self.solver
+        k = 2 * self.operator.degree + 1
+        out = {}
+        for b in self.operator.mesh.element_blocks:
+            t = b.element_type
+            W = s.conserved_to_primitive(states[t])
+            rho = np.maximum(W[..., 0], 1e-14)
+            pr = np.maximum(W[..., 4], 1e-14)
+            speed = np.linalg.norm(W[..., 1:4], axis=-1) + np.sqrt(s.gamma * pr / rho)
+            L = self.characteristic_lengths[t]
+            rate = k * speed.max(1) / L
+            if s.dynamic_viscosity > 0.0:
+                rmin = np.maximum(rho.mean(1), 0.1)
+                diff = np.maximum((4.0 / 3.0) * s.dynamic_viscosity / rmin,
+                                  s.thermal_conductivity / (rmin * s.specific_heat_cp))
+                pen = float(getattr(self.operator, "br2_stabilization", 1.0))
+                rate = rate + 2.0 * pen * k ** 2 * diff / np.maximum(L, 0.01) ** 2
+            out[t] = rate
+        return out
+
+    def local_dt(self, states, cfl=None):
+        """hybrid/time_integration.py:171-262: per-element cfl / rate, two
+        rounds of neighbour smoothing (ratio 1.3, each connection group reading
+        both sides before updating them, groups in order), then the sliver
+        factor min(1, V / 1e-4).  Returns {type: (E, 1, 1)}."""
+        c = self.cfl if cfl is None else float(cfl)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            dts = {t: c / np.maximum(r, 1e-12) for t, r in self._element_rates(states).items()}
+        alpha = 1.3
+        conns = self.internal_connections()
+        for _ in range(2):
+            for ot, oc, nt, nc in conns:
+                d_own = dts[ot][oc].copy()
+                d_nbr = dts[nt][nc].copy()
+                np.minimum.at(dts[ot], oc, alpha * d_nbr)
+                np.minimum.at(dts[nt], nc, alpha * d_own)
+        geo = self.operator.geometry
+        for bid, b in enumerate(self.operator.mesh.element_blocks):
+            vol = geo.element_metrics[bid].integration_weights.sum(1)
+            dts[b.element_type] = dts[b.element_type] * np.minimum(1.0, vol / 1.0e-4)
+        return {t: v[:, None, None] for t, v in dts.items()}
+
+    def conserved_totals(self, states):
+        """sum over elements of int_K U dV per variable (FP64)."""
+        geo = self.operator.geometry
+        tot = 0.0
+        for bid, b in enumerate(self.operator.mesh.element_blocks):
+            ref = geo.reference_elements[b.element_type]
+            w = geo.element_metrics[bid].integration_weights
+            tot = tot + np.einsum("qi,eiv,eq->v", ref.basis(ref.volume_points),
+                                  states[b.element_type], w)
+        return tot
+
+    def advance(self, states, target_time, boundary_state=None, coupler=None, lts=False,
+                cfl_interval=1, maximum_steps=10 ** 9, maximum_rejections=16):
+        """The step loop of run_hybrid_sphere.py:265-346 without soft start,
+        output or GUI: dt from stable_dt (or local_dt) every cfl_interval
+        steps, clipped to the target time, halved on a rejected step and
+        grown by 1.25 (up to the CFL value) on an accepted one; the chemistry
+        coupler after each accepted step.  Returns (states, time, steps,
+        rejections)."""
+        t, step, rej, consecutive, retry = 0.0, 0, 0, 0, 1.0
+        cached = None
+        while t < target_time and step < maximum_steps:
+            if cached is None or step % cfl_interval == 0:
+                cached = self.local_dt(states) if lts else self.stable_dt(states)
+            if lts:
+                dtd = {k: retry * v for k, v in cached.items()}
+                dt = float(min(float(v.min()) for v in dtd.values()))
+                adv = dtd
+            else:
+                dt = min(retry * cached, target_time - t)
+                adv = dt
+            if not np.isfinite(dt) or dt <= 0.0:
+                raise FloatingPointError(f"invalid timestep at step {step}: {dt}")
+            cand, ok, rmin, pmin = self.checked_step(states, adv, boundary_state)
+            if ok:
+                if coupler is not None:
+                    reacted = {k: coupler.react_block(v, adv[k] if lts else dt)
+                               for k, v in cand.items()}
+                    finite = all(np.all(np.isfinite(v)) for v in reacted.values())
+                    cand = reacted if finite else cand
+                states = cand
+                t += dt
+                step += 1
+                consecutive = 0
+                retry = min(1.0, 1.25 * retry)
+                if target_time - t <= 32.0 * np.finfo(np.float32).eps * max(abs(target_time),
+                                                                             1e-12):
+                    t = target_time
+            else:
+                rej += 1
+                consecutive += 1
+                retry *= 0.5
+                if consecutive >= maximum_rejections:
+                    raise FloatingPointError("Unable to find an admissible SSP-RK3 step")
+        return states, t, step, rej
+
     def state_admissibility(self, states):
         """hybrid/time_integration.py:291-312 (nodes only)."""
         finite = True

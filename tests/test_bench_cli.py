@@ -45,3 +45,38 @@ def test_face_traffic_is_the_algorithmic_traffic():
         algo = bench.face_bytes(64, 16, 5, itemsize) * faces
         measured = bench.measured_traffic("face", key)
         assert abs(measured / algo - 1.0) < 0.1, (key, measured, algo)
+
+
+def test_kernel_table_work_per_launch():
+    """The roofline's algorithmic work per launch: per interior face 2 N_f V B
+    of traces read + 4 Q_f B of geometry + 2 N_f V B written (one side on
+    boundary faces); per element the sum-factorised flops for hexahedra at
+    p >= 2 and the dense-operator flops otherwise."""
+    bench = _bench()
+    sys.path.insert(0, str(ROOT / "jax-fluid-sim_amd"))
+    from hybrid import HybridCompressibleSolver, HybridNavierStokesOperator, box_mesh
+    names = {"xmin": "inlet", "xmax": "outlet"}
+    m = box_mesh(3, split_fraction=0.4, seed=3, plane_names=names)
+    s = HybridCompressibleSolver(m, species_names=tuple(f"s{k}" for k in range(11)),
+                                 polynomial_degree=2, dynamic_viscosity=0.01, device="cpu")
+    op = HybridNavierStokesOperator(s)
+    dm = op.device_mesh
+    ng, nb = len(dm.groups), len(dm.block_tables)
+    timing = ([2.0] * ng, [4] * ng, [1.0] * nb, [2] * nb)
+    rows = bench.kernel_table(op, 2, 16, 4, timing)
+    assert len(rows) == ng + nb
+    by = {r["kernel"]: r for r in rows}
+    hexhex = by["face_kernel<quad,hex,hex,p2,V16,f32>"]
+    assert hexhex["bytes_per_launch"] == hexhex["count"] * (2 * 18 * 16 * 4 + 4 * 9 * 4)
+    bnd = by["face_kernel<quad,hex,boundary,p2,V16,f32>"]
+    assert bnd["bytes_per_launch"] == bnd["count"] * (2 * 9 * 16 * 4 + 4 * 9 * 4)
+    assert hexhex["avg_ms"] == 0.5
+    tet = by["element_kernel<tet,p2,V16,f32>"]
+    assert tet["flops_per_launch"] == tet["count"] * bench.element_flops(10, 125, 16)
+    hx = by["hex_sf_kernel<p2,V16,f32>"]
+    assert hx["flops_per_launch"] == hx["count"] * bench.hex_sf_flops(2, 16)
+    r = bench.roofline_of(tet, "f32", traffic=123)
+    assert r["bound"] == "mfma" and r["unit"] == "TFLOP/s" and r["traffic"] == 123
+    assert abs(r["frac"] - r["achieved"] / 157.3) < 1e-12
+    r = bench.roofline_of(hexhex, "f32")
+    assert r["bound"] == "hbm" and abs(r["achieved"] - hexhex["bytes_per_launch"] / 0.5e-3 / 1e9) < 1e-6
