@@ -374,6 +374,16 @@ int fsg_mood_troubled(fsg_mood* m, const void* const* U, void* stream);
  * alias U. */
 int fsg_mood_positivity(fsg_mood* m, const void* const* U, void* const* out, void* stream);
 
+/* HybridModalShockLimiter.__call__ (hybrid/limiter.py:186-264): on the
+ * elements the troubled-cell sensor flags (degree >= 2), the P1 projection
+ * with the element mean restored, or the mean where that projection's
+ * density or pressure at the samples drops below p0_factor x the floors or
+ * p0_relative x the mean's (or is non-finite); then the positivity limiter.
+ * out may alias U.  diag (may be null) receives the admissibility of the
+ * result (accumulated: reset it first with fsg_diag_reset). */
+int fsg_mood_modal(fsg_mood* m, const void* const* U, void* const* out, double p0_factor,
+                   double p0_relative, fsg_stage_diag* diag, void* stream);
+
 /* HybridMOODSubcellLimiter.recompute (hybrid/limiter.py:302-338): detection
  * on `detect` (or on dg_candidate when null) unless use_flags != 0 (then
  * the blocks' flags are taken as given), the subcell FV recompute of the

@@ -1036,7 +1036,9 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
   const int b = locate(a, g0 + (int64_t)blockIdx.x, &e);
   const TypeT<real>& T = a.types[a.btype[b]];
   const int N = T.N;
-  const real* src = mode == 0 ? a.det_src[b] : a.work[b];
+  // mode 0: limit det_src; 1: limit work, blend with base, limit again,
+  // diagnostics; 2: limit det_src, diagnostics
+  const real* src = mode == 1 ? a.work[b] : a.det_src[b];
   const real mwl = lane < N ? T.mw[lane] : real(0);
   load_rows<64, 8>(&sU[0][0], src + e * N * V, N * V, lane);
   if (lane < N) sW[lane] = mwl;
@@ -1067,7 +1069,7 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
   }
   real* out = a.out[b] + e * N * V;
   for (int k = lane; k < N * V; k += 64) out[k] = sU[k / V][k % V];
-  if (mode == 1 && a.diag) {
+  if (mode != 0 && a.diag) {
     // hybrid/time_integration.py:291-312 over the nodes
     real rmin = __builtin_huge_val(), pmin = __builtin_huge_val();
     bool nonfinite = false;
@@ -1091,6 +1093,119 @@ __global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real>
       if (nf) atomicAdd(&a.diag->nonfinite, 1u);
     }
   }
+}
+
+// HybridModalShockLimiter (hybrid/limiter.py:186-264), one wave per element:
+// the troubled-cell sensor; on a troubled element the P1 projection of every
+// variable with the element mean restored, or the mean itself where that
+// projection is non-finite or its density / pressure at the samples (nodes,
+// volume and face points) falls below p0_factor x the floors or p0_rel x the
+// mean's value.  Untroubled elements are copied.  Writes a.work.
+template <int V, typename real>
+__global__ __launch_bounds__(64) void modal_kernel(Args<real> a, Phys<real> ph, real p0_factor,
+                                                   real p0_rel, int64_t g0) {
+  __shared__ real sU[MAXN][V];
+  __shared__ real sL[MAXN][V];
+  __shared__ real sm[MAXN][2];
+  __shared__ real sMean[2][V];
+  const int lane = threadIdx.x;
+  int64_t e;
+  const int b = locate(a, g0 + (int64_t)blockIdx.x, &e);
+  const TypeT<real>& T = a.types[a.btype[b]];
+  const int N = T.N;
+  load_rows<64, 8>(&sU[0][0], a.det_src[b] + e * N * V, N * V, lane);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (lane < N) {
+    fsg::Prim<V, real> pr = fsg::c2p<V, real>(sU[lane], ph.gamma);
+    sm[lane][0] = pr.rho;
+    sm[lane][1] = pr.p;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  // the sensor of the detection kernel (hybrid/limiter.py:164-185)
+  real hi0 = 0, hi1 = 0, sc0 = 0, sc1 = 0;
+  if (lane < N) {
+    real l0 = 0, l1 = 0;
+    for (int j = 0; j < N; ++j) {
+      const real c = T.p1proj[j * N + lane];
+      l0 += c * sm[j][0];
+      l1 += c * sm[j][1];
+    }
+    const real m0 = sm[lane][0], m1 = sm[lane][1];
+    hi0 = (m0 - l0) * (m0 - l0);
+    hi1 = (m1 - l1) * (m1 - l1);
+    sc0 = m0 * m0;
+    sc1 = m1 * m1;
+  }
+  hi0 = gsum<64>(hi0);
+  hi1 = gsum<64>(hi1);
+  sc0 = gsum<64>(sc0);
+  sc1 = gsum<64>(sc1);
+  const real r0 = hi0 / (sc0 > real(1e-12) ? sc0 : real(1e-12));
+  const real r1 = hi1 / (sc1 > real(1e-12) ? sc1 : real(1e-12));
+  const bool troubled = a.sensor_on && nmax(r0, r1) > a.thr;
+  real* out = a.work[b] + e * N * V;
+  if (!troubled) {
+    for (int k = lane; k < N * V; k += 64) out[k] = sU[k / V][k % V];
+    return;
+  }
+  if (lane < N)
+    for (int v = 0; v < V; ++v) {
+      real acc = 0;
+      for (int j = 0; j < N; ++j) acc += T.p1proj[j * N + lane] * sU[j][v];
+      sL[lane][v] = acc;
+    }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (lane < V) {
+    real mo = 0, ml = 0;
+    for (int j = 0; j < N; ++j) {
+      mo += T.mw[j] * sU[j][lane];
+      ml += T.mw[j] * sL[j][lane];
+    }
+    sMean[0][lane] = mo / T.wsum;
+    sMean[1][lane] = ml / T.wsum;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (lane < N)
+    for (int v = 0; v < V; ++v) sL[lane][v] += sMean[0][v] - sMean[1][v];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  real rmin = __builtin_huge_val(), pmin = __builtin_huge_val();
+  bool fin = true;
+  for (int k = lane; k < N + T.NS; k += 64) {
+    real u[V];
+    if (k < N) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) u[v] = sL[k][v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) u[v] = real(0);
+      for (int j = 0; j < N; ++j) {
+        const real c = T.samples[(int64_t)j * T.NS + (k - N)];
+#pragma unroll
+        for (int v = 0; v < V; ++v) u[v] += c * sL[j][v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) fin = fin && isnum(u[v]);
+    real W[V];
+    fsg::c2p_full<V, real>(u, ph.gamma, W);
+    rmin = nmin(rmin, W[0]);
+    pmin = nmin(pmin, W[4]);
+  }
+  rmin = wmin(rmin);
+  pmin = wmin(pmin);
+  fin = !wany(!fin);
+  real Wm[V];
+  fsg::c2p_full<V, real>(sMean[0], ph.gamma, Wm);
+  const real md = Wm[0] > a.dfloor ? Wm[0] : a.dfloor;
+  const real mp = Wm[4] > a.pfloor ? Wm[4] : a.pfloor;
+  const bool severe = !fin || rmin < p0_factor * a.dfloor || pmin < p0_factor * a.pfloor ||
+                      rmin < p0_rel * md || pmin < p0_rel * mp;
+  for (int k = lane; k < N * V; k += 64) out[k] = severe ? sMean[0][k % V] : sL[k / V][k % V];
 }
 
 }  // namespace fsgmood
@@ -1388,6 +1503,32 @@ int troubled_impl(fsg_mood* m, const void* const* U, hipStream_t s) {
 }
 
 template <int V, typename real>
+int modal_impl(fsg_mood* m, const void* const* U, void* const* out, double p0_factor,
+               double p0_rel, fsg_stage_diag* diag, hipStream_t s) {
+  Args<real> a = base_args<real>(m);
+  set_ptrs(a.det_src, U, a.nblocks);
+  set_ptrs(a.work, (const void* const*)out, a.nblocks);
+  Phys<real> ph = phys_of<real>(m, nullptr);
+  if (a.sensor_on) {
+    for (size_t k = 0; k < m->offsets.size(); ++k) {
+      const int64_t count =
+          (k + 1 < m->offsets.size() ? m->offsets[k + 1] : m->total) - m->offsets[k];
+      if (count <= 0) continue;
+      hipLaunchKernelGGL((modal_kernel<V, real>), dim3((unsigned)count), dim3(64), 0, s, a, ph,
+                         real(p0_factor), real(p0_rel), m->offsets[k]);
+    }
+    if (int rc = launched("modal limiter")) return rc;
+    set_ptrs(a.det_src, (const void* const*)out, a.nblocks);
+  }
+  // then HybridPositivityLimiter (hybrid/limiter.py:264), with the
+  // admissibility of the result into diag when given
+  set_ptrs(a.out, (const void* const*)out, a.nblocks);
+  a.diag = reinterpret_cast<StageDiag*>(diag);
+  launch_positivity<V, real>(m, a, ph, diag ? 2 : 0, s);
+  return launched("positivity");
+}
+
+template <int V, typename real>
 int positivity_impl(fsg_mood* m, const void* const* U, void* const* out, hipStream_t s) {
   Args<real> a = base_args<real>(m);
   set_ptrs(a.det_src, U, a.nblocks);
@@ -1559,6 +1700,13 @@ int fsg_mood_positivity(fsg_mood* m, const void* const* U, void* const* out, voi
   if (!m || !U || !out) return fsg_fail(-10, "null argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   FSG_MOOD_DISPATCH(positivity_impl, (m, U, out, s));
+}
+
+int fsg_mood_modal(fsg_mood* m, const void* const* U, void* const* out, double p0_factor,
+                   double p0_relative, fsg_stage_diag* diag, void* stream) {
+  if (!m || !U || !out) return fsg_fail(-10, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FSG_MOOD_DISPATCH(modal_impl, (m, U, out, p0_factor, p0_relative, diag, s));
 }
 
 int fsg_mood_recompute(fsg_mood* m, const void* const* states, const void* const* dg_candidate,

@@ -7,7 +7,8 @@ kernels of libfluid_gfx950.so.
 
 from .boundary import FarfieldWallBoundary
 from .meshgen import box_mesh, smoke_mesh
-from .mood import HybridMOODSubcellLimiter, HybridPositivityLimiter, HybridTroubledCellSensor
+from .mood import (HybridModalShockLimiter, HybridMOODSubcellLimiter, HybridPositivityLimiter,
+                   HybridTroubledCellSensor)
 from .solver import CompressibleStateLayout, HybridCompressibleSolver
 from .spatial import HybridEulerOperator, HybridNavierStokesOperator
 from .time_integration import HybridSSPRK3
@@ -16,4 +17,4 @@ from .topology import Topology
 __all__ = ["FarfieldWallBoundary", "box_mesh", "smoke_mesh", "CompressibleStateLayout",
            "HybridCompressibleSolver", "HybridEulerOperator", "HybridNavierStokesOperator",
            "HybridSSPRK3", "Topology", "HybridMOODSubcellLimiter", "HybridPositivityLimiter",
-           "HybridTroubledCellSensor"]
+           "HybridTroubledCellSensor", "HybridModalShockLimiter"]

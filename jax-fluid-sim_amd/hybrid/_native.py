@@ -169,6 +169,8 @@ def lib():
         L.fsg_mood_stage_fused.argtypes = [vp, pp, pp, pp, pp, pp, pp, f64, pp, f64, f64, vp, vp,
                                            vp]
         L.fsg_mood_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), vp]
+        L.fsg_mood_modal.argtypes = [vp, pp, pp, f64, f64, vp, vp]
+        L.fsg_mood_modal.restype = ctypes.c_int
         for name in ("fsg_mood_create", "fsg_mood_troubled", "fsg_mood_positivity",
                      "fsg_mood_recompute", "fsg_mood_stage", "fsg_mood_stage_fused",
                      "fsg_mood_count"):

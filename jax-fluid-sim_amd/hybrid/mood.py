@@ -245,6 +245,45 @@ class HybridPositivityLimiter:
         return out
 
 
+class HybridModalShockLimiter:
+    """hybrid/limiter.py:186-264: the conservative P2-to-P1 (or P0) fallback
+    on sensor-flagged elements, then positivity; one device pass
+    (fsg_mood_modal)."""
+
+    def __init__(self, operator, threshold=5.0e-2, positivity=None, p0_fallback_factor=100.0,
+                 p0_relative_floor=0.2):
+        self.operator = operator
+        self.solver = operator.solver
+        self.threshold = float(threshold)
+        self.positivity = HybridPositivityLimiter(operator) if positivity is None else positivity
+        self.p0_fallback_factor = float(p0_fallback_factor)
+        self.p0_relative_floor = float(p0_relative_floor)
+        self._context_obj = None
+
+    def _ctx(self):
+        if self._context_obj is None:
+            pos = self.positivity
+            self._context_obj = _MoodContext(self.operator, self.threshold, pos.density_floor,
+                                             pos.pressure_floor, pos.relative_floor, "weno",
+                                             "reference")
+        return self._context_obj
+
+    def apply(self, states, out, diag=None):
+        """Limit ``states`` into ``out`` (may be the same dict); with ``diag``
+        (a device fsg_stage_diag) accumulate the admissibility of the
+        result."""
+        op = self.operator
+        nat.check(nat.lib().fsg_mood_modal(
+            self._ctx().ctx, nat.ptr_array(op._inputs(states)),
+            nat.ptr_array([out[t] for t in op.block_types]), self.p0_fallback_factor,
+            self.p0_relative_floor, diag, nat.stream_handle(self.solver.device)))
+        return out
+
+    def __call__(self, states):
+        out = {t: torch.empty_like(states[t]) for t in self.operator.block_types}
+        return self.apply(states, out)
+
+
 class HybridTroubledCellSensor:
     """hybrid/limiter.py:150-185: returns (sensor values, sensor > threshold)."""
 

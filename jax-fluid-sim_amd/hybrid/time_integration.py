@@ -36,8 +36,10 @@ class StageDiagnostics:
 class HybridSSPRK3:
     def __init__(self, operator, cfl=0.15, limiter=None, density_floor=1.0e-7,
                  pressure_floor=1.0e-7, maximum_rejections=16):
-        if limiter is not None and not hasattr(limiter, "recompute"):
-            raise NotImplementedError("only limiter=None or a MOOD limiter is supported")
+        if limiter is not None and not (hasattr(limiter, "recompute")
+                                        or hasattr(limiter, "apply")):
+            raise NotImplementedError("limiter must be None, a MOOD limiter or the modal "
+                                      "shock limiter")
         self.operator = operator
         self.solver = operator.solver
         self.cfl = float(cfl)
@@ -46,6 +48,7 @@ class HybridSSPRK3:
         self.pressure_floor = float(pressure_floor)
         self.maximum_rejections = int(maximum_rejections)
         self._diag = StageDiagnostics(self.solver.device)
+        self._scratch_diag = StageDiagnostics(self.solver.device)
         self._buffers = None
         # MOOD stages from one residual pass (inviscid and viscous together,
         # the viscous rhs re-evaluated on troubled elements only); False
@@ -242,10 +245,26 @@ class HybridSSPRK3:
         return U, host.time, int(host.step), int(host.rejections)
 
     def _stage(self, base, curr, nxt, dt, alpha, beta, boundary_state, reset):
+        """One fused residual + RK-epilogue stage into nxt; with a
+        limiter-after-stage (the modal shock limiter), the unlimited stage
+        goes to a temporary and the limiter writes nxt and the diagnostics
+        (hybrid/time_integration.py:432-439)."""
         op = self.operator
         stream = nat.stream_handle(self.solver.device)
         if reset:
             self._diag.reset(stream)
+        if self.limiter is not None and hasattr(self.limiter, "apply"):
+            tmp = {t: torch.empty_like(nxt[t]) for t in op.block_types}
+            self._scratch_diag.reset(stream)
+            self._stage_kernel(base, curr, tmp, dt, alpha, beta, boundary_state,
+                               self._scratch_diag)
+            self.limiter.apply(tmp, nxt, nat.ptr(self._diag.buf))
+            return
+        self._stage_kernel(base, curr, nxt, dt, alpha, beta, boundary_state, self._diag)
+
+    def _stage_kernel(self, base, curr, nxt, dt, alpha, beta, boundary_state, diag):
+        op = self.operator
+        stream = nat.stream_handle(self.solver.device)
         b = op._boundary(boundary_state)
         Ub, Uc = op._inputs(base), op._inputs(curr)
         Un = [nxt[t] for t in op.block_types]
@@ -257,7 +276,7 @@ class HybridSSPRK3:
         nat.check(nat.lib().fsg_rk_stage(op.device_mesh.ctx, nat.ptr_array(Ub), nat.ptr_array(Uc),
                                          nat.ptr_array(Un), float(dt), dt_elem, float(alpha),
                                          float(beta), op.rhs_terms(), ctypes.byref(b),
-                                         nat.ptr(self._diag.buf), stream))
+                                         nat.ptr(diag.buf), stream))
 
     def _mood_stage(self, base, curr, nxt, dt, alpha, beta, boundary_state):
         """The MOOD branch of hybrid/time_integration.py:404-439: inviscid and
@@ -285,11 +304,14 @@ class HybridSSPRK3:
         self.limiter.stage(base, curr, inv, visc, work, nxt, dt, alpha, beta, boundary_state,
                            nat.ptr(self._diag.buf))
 
+    def _mood(self):
+        return self.limiter is not None and hasattr(self.limiter, "recompute")
+
     def rk_stage(self, base_states, current_states, dt, base_weight, current_weight,
                  boundary_state=None):
         """hybrid/time_integration.py:404-439."""
         nxt = {t: torch.empty_like(current_states[t]) for t in self.operator.block_types}
-        if self.limiter is not None:
+        if self._mood():
             self._mood_stage(base_states, current_states, nxt, dt, base_weight, current_weight,
                              boundary_state)
         else:
@@ -302,7 +324,7 @@ class HybridSSPRK3:
         stages, valid = all stages valid, minima over the stages.  With MOOD
         (:463-554): three MOOD stages, diagnostics of the last one."""
         types = self.operator.block_types
-        if self.limiter is not None:
+        if self._mood():
             s = states
             for a, b in RK3_WEIGHTS:
                 nxt = {t: torch.empty_like(states[t]) for t in types}

@@ -124,6 +124,51 @@ class TroubledCellSensor:
         return sensor, {k: v > self.threshold for k, v in sensor.items()}
 
 
+class ModalShockLimiter:
+    """hybrid/limiter.py:186-264: on sensor-flagged elements (degree >= 2)
+    the P1 projection of every variable with the element mean restored, or
+    the mean itself where that projection is non-finite at the samples or
+    its density / pressure there drops below p0_fallback_factor x the floors
+    or p0_relative_floor x the mean's; then the positivity limiter."""
+
+    def __init__(self, operator, threshold=5.0e-2, positivity=None, p0_fallback_factor=100.0,
+                 p0_relative_floor=0.2):
+        self.operator = operator
+        self.sensor = TroubledCellSensor(operator, threshold)
+        self.positivity = PositivityLimiter(operator) if positivity is None else positivity
+        self.p0_fallback_factor = float(p0_fallback_factor)
+        self.p0_relative_floor = float(p0_relative_floor)
+
+    def __call__(self, states):
+        if self.operator.degree < 2:
+            return self.positivity(states)
+        _, troubled = self.sensor(states)
+        pos = self.positivity
+        s = self.operator.solver
+        out = {}
+        for bid, b in enumerate(self.operator.mesh.element_blocks):
+            t = b.element_type
+            U = states[t]
+            low = np.einsum("ij,ejv->eiv", self.sensor.p1_projectors[t], U)
+            old = pos.mean(bid, U)
+            low = low + (old - pos.mean(bid, low))[:, None, :]
+            smp = pos.sample(t, low)
+            with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+                W = s.conserved_to_primitive(smp)
+                dmin, pmin = W[..., 0].min(1), W[..., 4].min(1)
+                Wm = s.conserved_to_primitive(old)
+            md = np.maximum(Wm[..., 0], pos.density_floor)
+            mp = np.maximum(Wm[..., 4], pos.pressure_floor)
+            finite = np.all(np.isfinite(smp), axis=(1, 2))
+            severe = (~finite | (dmin < self.p0_fallback_factor * pos.density_floor)
+                      | (pmin < self.p0_fallback_factor * pos.pressure_floor)
+                      | (dmin < self.p0_relative_floor * md)
+                      | (pmin < self.p0_relative_floor * mp))
+            fb = np.where(severe[:, None, None], old[:, None, :], low)
+            out[t] = np.where(troubled[t][:, None, None], fb, U)
+        return pos(out)
+
+
 class MOODSubcellLimiter:
     """hybrid/limiter.py:267-343."""
 

@@ -301,3 +301,32 @@ def test_fused_stage_matches_split_stage(dtype):
     assert sum(int(f.sum()) for f in flags[True].values()) > 0
     tol = 1e-13 if dtype == torch.float64 else 1e-5
     assert np.all(rel_l2(out[True], out[False]) <= tol), rel_l2(out[True], out[False])
+
+
+@pytest.mark.parametrize("name,p", [("split", 2), ("split", 3), ("sliver", 2)])
+def test_modal_shock_limiter_matches(name, p):
+    """HybridModalShockLimiter (hybrid/limiter.py:186-264) against the oracle,
+    alone and as the limiter of an SSP-RK3 step (the stage candidate limited,
+    admissibility of the limited state)."""
+    from hybrid import HybridModalShockLimiter
+    from oracle.limiter import ModalShockLimiter
+    print(f"case modal {name} p={p}", flush=True)
+    solver, op, b, lim, oop, ob, olim, states = _case(name, p)
+    opos = PositivityLimiter(oop, density_floor=1e-7, pressure_floor=1e-7)
+    omod = ModalShockLimiter(oop, positivity=opos)
+    dmod = HybridModalShockLimiter(op, positivity=HybridPositivityLimiter(
+        op, density_floor=1e-7, pressure_floor=1e-7))
+    sensor, flags = omod.sensor(states)
+    for t, sv in sensor.items():
+        assert np.all(np.abs(sv - 0.05) > 1e-9), t
+    assert sum(int(f.sum()) for f in flags.values()) > 0
+    want = omod(states)
+    got = _np(dmod(_dev(solver, states)))
+    assert np.all(rel_l2(got, want) <= _bar(omod, states)), rel_l2(got, want)
+    dt = _dt(name) / 2
+    ork = SSPRK3(oop, limiter=omod)
+    want, wv, wr, wp = ork.checked_step(states, dt, ob)
+    got, gv, gr, gp = HybridSSPRK3(op, limiter=dmod).checked_step(_dev(solver, states), dt, b)
+    assert bool(gv) == bool(wv)
+    assert abs(float(gr) - wr) <= 1e-12 * abs(wr)
+    assert np.all(rel_l2(_np(got), want) <= 1e-11), rel_l2(_np(got), want)
