@@ -15,6 +15,7 @@
 #ifndef FLUID_GFX950_H
 #define FLUID_GFX950_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -210,6 +211,23 @@ const char* fsg_last_error(void);
  * of the operand / accumulator lane layout). */
 int fsg_mfma_selftest(int32_t dtype, const void* A, const void* B, void* C, void* stream);
 int32_t fsg_abi_version(void);
+
+/* ---- XLA custom-call target -------------------------------------------------
+ * fsg_xla_rhs is the untyped GPU custom call
+ *   void f(hipStream_t stream, void** buffers, const char* opaque, size_t len)
+ * for jax.ffi.register_ffi_target(name, capsule, platform="ROCM", api_version=0),
+ * replacing HybridNavierStokesOperator.rhs / residual at the reference's call
+ * sites (run_hybrid_sphere.py:200-216; hybrid/navier_stokes.py:338).  opaque:
+ * "ctx=<hex>;bnd=<hex or 0>;mode=<0|1>;terms=<1..3>;nblocks=<n>"; buffers =
+ * [U_0 .. U_{n-1}, out_0 .. out_{n-1}].  Errors go to fsg_last_error() (the
+ * signature has no status).  fsg_xla_parse decodes an opaque string. */
+typedef struct {
+  fsg_context* ctx;
+  const fsg_boundary* boundary;
+  int32_t mode, terms, nblocks, reserved;
+} fsg_xla_rhs_config;
+int fsg_xla_parse(const char* opaque, size_t len, fsg_xla_rhs_config* cfg);
+void fsg_xla_rhs(void* stream, void** buffers, const char* opaque, size_t len);
 
 /* ---- Time-step control ------------------------------------------------------
  * The CFL estimate HybridSSPRK3.stable_dt (hybrid/time_integration.py:93-169,

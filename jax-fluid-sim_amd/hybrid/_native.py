@@ -96,6 +96,12 @@ class StepState(ctypes.Structure):
                                       "accept", "have_dt", "refresh_interval", "reserved")]
 
 
+class XlaRhsConfig(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("boundary", ctypes.c_void_p), ("mode", ctypes.c_int32),
+                ("terms", ctypes.c_int32), ("nblocks", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class Boundary(ctypes.Structure):
     _fields_ = [("has_boundary", ctypes.c_int32), ("has_wall_temperature", ctypes.c_int32),
                 ("wall_temperature", ctypes.c_double), ("farfield", ctypes.c_double * 16)]
@@ -170,6 +176,11 @@ def lib():
                                            vp]
         L.fsg_mood_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), vp]
         L.fsg_mood_modal.argtypes = [vp, pp, pp, f64, f64, vp, vp]
+        L.fsg_xla_parse.argtypes = [ctypes.c_char_p, ctypes.c_size_t,
+                                    ctypes.POINTER(XlaRhsConfig)]
+        L.fsg_xla_parse.restype = ctypes.c_int
+        L.fsg_xla_rhs.argtypes = [vp, pp, ctypes.c_char_p, ctypes.c_size_t]
+        L.fsg_xla_rhs.restype = None
         L.fsg_mood_modal.restype = ctypes.c_int
         for name in ("fsg_mood_create", "fsg_mood_troubled", "fsg_mood_positivity",
                      "fsg_mood_recompute", "fsg_mood_stage", "fsg_mood_stage_fused",

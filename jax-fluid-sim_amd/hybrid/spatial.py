@@ -73,6 +73,15 @@ class HybridEulerOperator:
     def rhs_terms(self):
         return nat.TERM_EULER
 
+    def xla_opaque(self, boundary_state=None, mode=nat.MODE_RHS, terms=None):
+        """The opaque string of the fsg_xla_rhs custom call for this operator
+        (the boundary struct it names is kept alive by the operator)."""
+        b = self._boundary(boundary_state)
+        self._xla_boundary = b
+        t = self.rhs_terms() if terms is None else int(terms)
+        return (f"ctx={self.device_mesh.ctx.value:x};bnd={ctypes.addressof(b):x};mode={int(mode)};"
+                f"terms={t};nblocks={len(self.block_types)}").encode()
+
     # -- reference API -----------------------------------------------------------
     def residual(self, states, boundary_state=None):
         """Weak-form Euler residual before M^-1 (hybrid/spatial.py:137-234)."""

@@ -248,3 +248,21 @@ def test_slip_overrides_no_slip_on_shared_tags():
     ref = oop.viscous_residual(states, ob)
     bar = _bar(lambda s: oop.viscous_residual(s, ob), states, VISCOUS_TOL)
     assert np.all(rel_l2(got, ref) <= bar), (rel_l2(got, ref), bar)
+
+
+def test_xla_custom_call_target_is_rhs():
+    """fsg_xla_rhs, called the way XLA calls an untyped custom-call target
+    (stream, buffer pointer array, opaque string), writes the operator's rhs
+    bit for bit."""
+    import ctypes
+    from hybrid import _native as nat
+    ds, dop, db, oop, ob, states = _case("split", 2)
+    U = ds.to_device(states)
+    want = dop.rhs(U, db)
+    outs = {t: torch.full_like(U[t], float("nan")) for t in dop.block_types}
+    bufs = nat.ptr_array([U[t] for t in dop.block_types] + [outs[t] for t in dop.block_types])
+    text = dop.xla_opaque(db)
+    nat.lib().fsg_xla_rhs(nat.stream_handle(ds.device), bufs, text, len(text))
+    torch.cuda.synchronize()
+    for t in dop.block_types:
+        assert torch.equal(outs[t], want[t]), t

@@ -113,3 +113,28 @@ def test_surrogate_inference_rejects_overlapping_output():
                                ctypes.c_double(1e-7), ctypes.c_void_p(base + off), one,
                                ctypes.c_int64(n), 0, None)
         assert rc == -11 and b"overlaps" in lib.fsg_mlp_last_error()
+
+
+def test_xla_custom_call_opaque():
+    """The custom-call target's opaque string: the operator's own string
+    decodes to its context, boundary, mode, terms and block count; malformed
+    or out-of-range strings are refused with a message."""
+    import ctypes
+    from conftest import PLANE_NAMES
+    from hybrid import FarfieldWallBoundary, HybridCompressibleSolver, \
+        HybridNavierStokesOperator, box_mesh
+    from hybrid import _native as nat
+    m = box_mesh(3, split_fraction=0.4, seed=3, plane_names=PLANE_NAMES)
+    s = HybridCompressibleSolver(m, polynomial_degree=2, dynamic_viscosity=0.01, device="cpu")
+    op = HybridNavierStokesOperator(s)
+    b = FarfieldWallBoundary(s, np.array([1.2, 0.9, 0.1, -0.05, 1.1]))
+    text = op.xla_opaque(b)
+    cfg = nat.XlaRhsConfig()
+    assert nat.lib().fsg_xla_parse(text, len(text), ctypes.byref(cfg)) == 0
+    assert cfg.ctx == op.device_mesh.ctx.value
+    assert cfg.boundary == ctypes.addressof(op._xla_boundary)
+    assert (cfg.mode, cfg.terms, cfg.nblocks) == (1, 3, 3)
+    for bad in (b"ctx=0;bnd=0;mode=1;terms=3;nblocks=3", b"ctx=12;bnd=0;mode=1;terms=3",
+                b"ctx=12;bnd=0;mode=7;terms=3;nblocks=3", b"ctx=zz;bnd=0;mode=1;terms=3;nblocks=3"):
+        assert nat.lib().fsg_xla_parse(bad, len(bad), ctypes.byref(cfg)) == -11
+        assert b"opaque" in nat.lib().fsg_last_error()
