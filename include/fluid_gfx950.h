@@ -498,6 +498,21 @@ int fsg_chem_integrate_flagged_noalloc(const double* table, const double* rho_in
                                        const int32_t* flag, double* rho_out, double* T_out,
                                        int64_t n, void* stream);
 
+/* The integrator's judgement of a predicted substep (the surrogate's error
+ * check): the backward-Euler residual R = rho_pred - rho_0 - dt w(rho_pred, T)
+ * of PointImplicitIntegrator.step (hybrid/chemistry/integrator.py:31-104;
+ * rho_0 = max(rho_in, 1e-20); T recovered from rho_pred at the initial
+ * internal energy unless constant_temperature), one rate evaluation without
+ * a Newton solve, scaled by the Jacobian diagonal into the error estimate
+ * |R_s| / max(|1 - dt dw_s / drho_s|, 1).  flag[c] is set to 1 where the estimate
+ * exceeds rtol |rho_pred,s - rho_0,s| + atol sum(rho_0) for any species (and
+ * left as it is elsewhere); err_out (may be null) receives the largest
+ * estimate relative to sum(rho_0). */
+int fsg_chem_check(const double* table, const double* rho_in, const double* T_in,
+                   const double* dt_cell, double dt, int32_t constant_temperature,
+                   const double* rho_pred, double rtol, double atol, int32_t* flag,
+                   double* err_out, int64_t n, void* stream);
+
 /* The coupler's steps around its integrator call, for another substep in
  * between (the surrogate): prepare = species clip and rescale to rho, thermal
  * energy, T0 recovery (hybrid/chemistry/coupling.py:116-154) giving the

@@ -236,6 +236,98 @@ __device__ __forceinline__ void rates(const double* tab, const double* rho, doub
   }
 }
 
+// w = net mass production rates and the diagonal d w_s / d rho_s of their
+// Jacobian at fixed (T, Tv): the arithmetic of rates() without the
+// off-diagonal entries (hybrid/chemistry/mechanism.py:159-220).
+__device__ __forceinline__ void rates_diag(const double* tab, const double* rho, double T,
+                                           double* w, double* dd) {
+  const double* mw = tab;
+  const double* rx = tab + OFF_RX;
+  double C[NS], dC[NS], lc[NS], dlc[NS], g[NS];
+  const double tT = dmax(T, 10.0);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double x = rho[s] / mw[s];
+    C[s] = dmax(x, 0.0);
+    dC[s] = dmax_dx(x, 0.0) / mw[s];
+    double cs = dmax(C[s], 1.0e-100);
+    lc[s] = log(cs);
+    dlc[s] = dmax_dx(C[s], 1.0e-100) / cs;
+    Thermo a = nasa(tab + OFF_NASA + 17 * s, tT);
+    g[s] = a.h - a.s;
+  }
+  double om[NS], dom[NS], csum[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) om[s] = dom[s] = csum[s] = 0.0;
+  for (int r = 0; r < NR; ++r) {
+    const double A = rx[6 * r + 0], b = rx[6 * r + 1], th = rx[6 * r + 2];
+    const bool third = rx[6 * r + 3] != 0.0;
+    const double* n1 = tab + OFF_NU1 + r * NS;
+    const double* n2 = tab + OFF_NU2 + r * NS;
+    const double* al = tab + OFF_ALPHA + r * NS;
+    // T = Tv: every temperature mode is T (sqrt(T Tv) = T for T >= 1)
+    const double tf = dmax(select_temperature((int)rx[6 * r + 4], T, T), 10.0);
+    const double kf = A * pow(tf, b) * exp(-th / tf);
+    const double tr = dmax(select_temperature((int)rx[6 * r + 5], T, T), 10.0);
+    double dG = 0.0, dn = 0.0, lf = 0.0, lr = 0.0, Meff = 0.0;
+    if (tr == tT) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) dG += (n2[s] - n1[s]) * g[s];
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        Thermo a = nasa(tab + OFF_NASA + 17 * s, tr);
+        dG += (n2[s] - n1[s]) * (a.h - a.s);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      dn += n2[s] - n1[s];
+      lf += n1[s] * lc[s];
+      lr += n2[s] * lc[s];
+      Meff += al[s] * C[s];
+    }
+    const double Kc = exp(-clampd(dG, -700.0, 700.0)) * pow(P_STD / (R_UNIV * tr), dn);
+    const double kr = kf / dmax(Kc, 1.0e-300);
+    const double fwd = exp(clampd(lf, -700.0, 700.0));
+    const double rev = exp(clampd(lr, -700.0, 700.0));
+    const double cf = fabs(lf) < 700.0 ? 1.0 : (fabs(lf) == 700.0 ? 0.5 : 0.0);
+    const double cr = fabs(lr) < 700.0 ? 1.0 : (fabs(lr) == 700.0 ? 0.5 : 0.0);
+    const double col = third ? Meff : 1.0;
+    const double q = kf * fwd * col - kr * rev * col;
+    double mr = 0.0;   // mass change of the reaction, sum_k M_k dnu_k
+#pragma unroll
+    for (int s = 0; s < NS; ++s) mr += mw[s] * (n2[s] - n1[s]);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const double d = n2[s] - n1[s];
+      const double dfw = fwd * cf * n1[s] * dlc[s];
+      const double drv = rev * cr * n2[s] * dlc[s];
+      const double dcol = third ? al[s] : 0.0;
+      const double dq = (kf * (dfw * col + fwd * dcol) - kr * (drv * col + rev * dcol)) * dC[s];
+      om[s] += d * q;
+      dom[s] += d * dq;
+      csum[s] += mr * dq;
+    }
+  }
+  double srho = 0.0, S = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    srho += rho[s];
+    S += om[s] * mw[s];
+  }
+  const double tot = dmax(srho, 1.0e-14), dtot = dmax_dx(srho, 1.0e-14);
+  // w = M omega - Y sum(M omega) and its diagonal derivative (the mass-sum
+  // correction differentiated as in rates())
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double Y = dmax(rho[s], 0.0) / tot;
+    w[s] = om[s] * mw[s] - Y * S;
+    const double dY = dmax_dx(rho[s], 0.0) / tot - dmax(rho[s], 0.0) / (tot * tot) * dtot;
+    dd[s] = mw[s] * dom[s] - dY * S - Y * csum[s];
+  }
+}
+
 // Solve A x = b (A overwritten) by LU with partial pivoting, all unrolled.
 __device__ __forceinline__ void solve(double (*A)[NS], double* b) {
 #pragma unroll
@@ -455,6 +547,56 @@ __global__ __launch_bounds__(64) void finish_kernel(ureal* __restrict__ U, int64
   for (int s = 0; s < NS; ++s) u[5 + s] = (ureal)(rs[s] * f);
 }
 
+// The integrator's judgement of a predicted substep: the backward-Euler
+// residual R = rho_p - rho_0 - dt w(rho_p, T_p) of the scheme
+// hybrid/chemistry/integrator.py:31-104 solves (rho_0 = max(rho_in, 1e-20);
+// T_p recovered from rho_p at the initial internal energy unless the
+// temperature is held), scaled by the Jacobian diagonal into the one-step
+// error estimate |R_s| / max(|1 - dt dw_s/drho_s|, 1) (a stiff, decaying
+// species' residual is divided down; no residual is amplified).  A cell whose estimate
+// exceeds rtol |rho_p,s - rho_0,s| + atol sum(rho_0) for any species is
+// flagged (flag |= 1) for the integrator.
+__global__ __launch_bounds__(64) void check_kernel(const double* __restrict__ tab,
+                                                   const double* __restrict__ rho_in,
+                                                   const double* __restrict__ T_in,
+                                                   const double* __restrict__ dt_cell, double dt,
+                                                   int const_T, const double* __restrict__ rho_p,
+                                                   double rtol, double atol,
+                                                   int32_t* __restrict__ flag,
+                                                   double* __restrict__ err_out, int64_t n) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double r0[NS], rp[NS], Y0[NS], total = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    r0[s] = dmax(rho_in[c * NS + s], 1.0e-20);
+    rp[s] = rho_p[c * NS + s];
+    total += r0[s];
+  }
+  double T = T_in[c];
+  if (!const_T) {
+    const double tot = dmax(total, 1.0e-14);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) Y0[s] = r0[s] / tot;
+    T = recover_temperature(tab, rp, mixture_energy(tab, Y0, T), T, 10);
+  }
+  const double h = dt_cell ? dt_cell[c] : dt;
+  double w[NS], dd[NS];
+  rates_diag(tab, rp, T, w, dd);
+  double worst = 0.0;
+  bool bad = false;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double R = rp[s] - r0[s] - h * w[s];
+    const double est = fabs(R) / dmax(fabs(1.0 - h * dd[s]), 1.0);
+    const double tol = rtol * fabs(rp[s] - r0[s]) + atol * total;
+    bad = bad || !(est <= tol);
+    worst = dmax(worst, est / dmax(total, 1.0e-300));
+  }
+  if (bad) flag[c] = 1;
+  if (err_out) err_out[c] = worst;
+}
+
 // Integrator on the flagged cells of a batch only (the surrogate's
 // fallback), without a host round trip: flagged cell ids are compacted with
 // wavefront ballots into list[0..*count), then a bounded grid strides over
@@ -602,6 +744,18 @@ int fsg_chem_integrate_flagged_noalloc(const double* table, const double* rho_in
   return fsg_chem_integrate_flagged(table, rho_in, T_in, dt_cell, dt, newton_iters,
                                     constant_temperature, flag, ptr, ptr + FSG_NOALLOC_MAX_CELLS,
                                     rho_out, T_out, n, stream);
+}
+
+int fsg_chem_check(const double* table, const double* rho_in, const double* T_in,
+                   const double* dt_cell, double dt, int32_t constant_temperature,
+                   const double* rho_pred, double rtol, double atol, int32_t* flag,
+                   double* err_out, int64_t n, void* stream) {
+  if (!table || !rho_in || !T_in || !rho_pred || !flag) return chem_fail(-10, "null argument");
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(fsgchem::check_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), table, rho_in, T_in, dt_cell, dt,
+                     constant_temperature, rho_pred, rtol, atol, flag, err_out, n);
+  return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
 }
 
 int fsg_chem_prepare(const double* table, const void* U, int32_t dtype, int64_t ncells,

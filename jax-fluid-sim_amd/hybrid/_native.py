@@ -154,6 +154,8 @@ def lib():
         L.fsg_chem_integrate_flagged_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_double, i32,
                                                          i32, vp, vp, vp, i64, vp]
         L.fsg_chem_integrate_flagged_noalloc.restype = ctypes.c_int
+        L.fsg_chem_check.argtypes = [vp, vp, vp, vp, f64, i32, vp, f64, f64, vp, vp, i64, vp]
+        L.fsg_chem_check.restype = ctypes.c_int
         L.fsg_chem_prepare.argtypes = [vp, vp, i32, i64, ctypes.POINTER(ChemParams), vp, vp, vp]
         L.fsg_chem_prepare.restype = ctypes.c_int
         L.fsg_chem_finish.argtypes = [vp, i32, i64, ctypes.POINTER(ChemParams), vp, vp]

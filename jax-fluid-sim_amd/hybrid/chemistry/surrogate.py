@@ -22,9 +22,12 @@ with the temperature re-solved inside every Newton iteration
 far-from-equilibrium air moves by O(10 %) under 1e-16 input noise, a map no
 regression can follow; that mode can still be selected.
 
-A cell whose features leave the training hull (plus a margin) or whose
-projection misses the tolerance is flagged and advanced by the plain
-integrator instead, which therefore also measures the surrogate's error.
+A cell whose features leave the training hull (plus a margin), whose
+projection misses the tolerance, or whose prediction fails the integrator's
+own check -- the backward-Euler residual of the prediction, one rate
+evaluation scaled by the Jacobian diagonal into an error estimate, above
+check_rtol x the predicted change + check_atol x the cell density -- is
+advanced by the plain integrator instead.
 """
 
 import ctypes
@@ -104,9 +107,17 @@ class ChemistrySurrogate:
     the plain integrator's result where the cell was flagged."""
 
     def __init__(self, integrator, constant_temperature=True, seed=0, eps_y=1e-12,
-                 hull_margin=0.25, proj_tol=1e-9, device=None):
+                 hull_margin=0.25, proj_tol=1e-9, device=None, check_rtol=0.1,
+                 check_atol=1e-8):
         self.integrator = integrator
         self.constant_temperature = bool(constant_temperature)
+        # the integrator's error check of each prediction (fsg_chem_check):
+        # fall back where the estimated error of a species exceeds
+        # check_rtol x its predicted change + check_atol x the cell density
+        # (check_rtol None: no check)
+        self.check_rtol = None if check_rtol is None else float(check_rtol)
+        self.check_atol = float(check_atol)
+        self.last_error = None
         self.device = integrator.mechanism.device if device is None else torch.device(device)
         self.eps_y = float(eps_y)
         self.hull_margin = float(hull_margin)
@@ -312,8 +323,14 @@ class ChemistrySurrogate:
             dtt = dt.to(self.device, torch.float64).reshape(n).contiguous()
         else:
             dts = float(dt)
-        work = torch.empty(n + 1, dtype=torch.int32, device=self.device)
         integ = self.integrator
+        if self.check_rtol is not None:
+            self.last_error = torch.empty(n, dtype=torch.float64, device=self.device)
+            nat.check_chem(nat.lib().fsg_chem_check(
+                nat.ptr(integ.mechanism.table), nat.ptr(rho), nat.ptr(T), nat.ptr(dtt), dts,
+                int(self.constant_temperature), nat.ptr(out), self.check_rtol, self.check_atol,
+                nat.ptr(flag), nat.ptr(self.last_error), n, nat.stream_handle(self.device)))
+        work = torch.empty(n + 1, dtype=torch.int32, device=self.device)
         nat.check_chem(nat.lib().fsg_chem_integrate_flagged(
             nat.ptr(integ.mechanism.table), nat.ptr(rho), nat.ptr(T), nat.ptr(dtt), dts,
             integ.max_newton_iters, int(self.constant_temperature), nat.ptr(flag),

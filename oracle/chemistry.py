@@ -336,3 +336,26 @@ class ReactingFlowCoupler:
     def advance(self, states, dt):
         """hybrid/chemistry/coupling.py:199-209."""
         return {k: self.react_block(v, dt) for k, v in states.items()}
+
+
+def be_check(mech, rho_in, T_in, dt, rho_pred, constant_temperature=False, rtol=0.1,
+             atol=1e-8):
+    """The integrator's judgement of a predicted substep: the backward-Euler
+    residual R = rho_p - rho_0 - dt w(rho_p, T_p) of integrator_step's scheme
+    (hybrid/chemistry/integrator.py:31-104), T_p recovered from rho_p at the
+    initial internal energy (10 iterations from T_in) unless the temperature
+    is held, scaled by the Jacobian diagonal: est_s = |R_s| / max(|1 - dt
+    dw_s/drho_s|, 1).  Returns (flag, max_s est_s / sum(rho_0)); flag where est_s >
+    rtol |rho_p,s - rho_0,s| + atol sum(rho_0) for some s."""
+    r0 = np.maximum(np.asarray(rho_in, float), 1e-20)
+    rp = np.asarray(rho_pred, float)
+    total = r0.sum()
+    T = float(T_in)
+    if not constant_temperature:
+        e0 = mech.mixture_properties(r0, T)["internal_energy"]
+        T = mech.recover_temperature_from_energy(rp, e0, T_init=T, num_iters=10)
+    _, w, dw = mech.net_production_rates(rp, T, T, jacobian=True)
+    R = rp - r0 - dt * w
+    est = np.abs(R) / np.maximum(np.abs(1.0 - dt * np.diag(dw)), 1.0)
+    tol = rtol * np.abs(rp - r0) + atol * total
+    return bool(np.any(~(est <= tol))), float(est.max() / total)

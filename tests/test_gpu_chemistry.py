@@ -213,3 +213,47 @@ def test_coupler_reference_state():
         U[e, :, 5:] = rho * Y
     got = _check_coupler(U, 1e-7, 16, False, 0.45)
     assert got[0, 0, 5 + 7] > 1e-10           # O2 dissociates at 6000 K
+
+
+@pytest.mark.parametrize("const_T", [True, False])
+def test_integrator_check_matches_oracle(const_T):
+    """fsg_chem_check (the backward-Euler residual of a prediction, scaled by
+    the Jacobian diagonal) against the oracle's be_check: the error
+    estimates to 1e-9 relative and identical flags (cells within 1e-6 of
+    the bar excluded; the test asserts most cells are compared)."""
+    import ctypes
+    from hybrid import _native as nat
+    from oracle.chemistry import be_check
+    # air at 3000-5000 K over 1e-6 s: the 12-iteration scheme converges (its
+    # own result has a residual at round-off) and the predictions, the
+    # integrator's changes with 5 % noise, are partly flagged
+    rho, T = air_batch(64, seed=3, T_range=(3000.0, 5000.0))
+    om = Mechanism()
+    dt = 1e-6
+    good = np.array([integrator_step(om, rho[c], T[c], dt, 12, const_T)[0] for c in range(64)])
+    rng = np.random.default_rng(2)
+    r0 = np.maximum(rho, 1e-20)
+    pred = r0 + (good - r0) * (1 + 0.05 * rng.standard_normal(good.shape))
+    assert not any(be_check(om, rho[c], T[c], dt, good[c], const_T)[0] for c in range(64))
+    mech = Air11ParkMechanism()
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    flag = torch.zeros(64, dtype=torch.int32, device="cuda")
+    err = torch.empty(64, dtype=torch.float64, device="cuda")
+    Rd, Td, Pd = dev(rho), dev(T), dev(pred)
+    nat.check_chem(nat.lib().fsg_chem_check(nat.ptr(mech.table), nat.ptr(Rd), nat.ptr(Td), None,
+                                            dt, int(const_T), nat.ptr(Pd), 0.1, 1e-8,
+                                            nat.ptr(flag), nat.ptr(err), 64,
+                                            nat.stream_handle(Rd.device)))
+    got_f, got_e = flag.cpu().numpy().astype(bool), err.cpu().numpy()
+    want = [be_check(om, rho[c], T[c], dt, pred[c], const_T) for c in range(64)]
+    want_f = np.array([w[0] for w in want])
+    want_e = np.array([w[1] for w in want])
+    np.testing.assert_allclose(got_e, want_e, rtol=1e-9)
+    # cells far enough from the bar to have an unambiguous flag
+    margin = np.array([
+        be_check(om, rho[c], T[c], dt, pred[c], const_T, 0.1 * (1 - 1e-6), 1e-8 * (1 - 1e-6))[0]
+        == be_check(om, rho[c], T[c], dt, pred[c], const_T, 0.1 * (1 + 1e-6), 1e-8 * (1 + 1e-6))[0]
+        for c in range(64)])
+    print(f"check: flagged {want_f.mean():.2f}, compared {margin.mean():.2f}", flush=True)
+    assert margin.mean() > 0.9 and 0 < want_f.mean() < 1
+    assert np.array_equal(got_f[margin], want_f[margin])

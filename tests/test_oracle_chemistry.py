@@ -159,3 +159,29 @@ def test_device_table_transcription_matches_oracle():
         assert np.array_equal(tab[o:o + nr * ns].reshape(nr, ns), arr)
         o += nr * ns
     assert o == len(tab)
+
+
+def test_be_check_pins():
+    """The integrator's check of a prediction: its own converged isothermal
+    result passes with a negligible error estimate; the unchanged input
+    (identity prediction) has R = -dt w(rho_0) exactly; a prediction whose
+    change of a reacting species is off by more than the relative bar is
+    flagged."""
+    from oracle.chemistry import be_check
+    rho0 = np.zeros(11)
+    rho0[9], rho0[10], rho0[7] = 0.7, 0.25, 0.05
+    T, dt = 5000.0, 1e-7
+    good, _, _ = integrator_step(MECH, rho0, T, dt, iters=16, constant_temperature=True)
+    flag, err = be_check(MECH, rho0, T, dt, good, constant_temperature=True)
+    assert not flag and err < 1e-12
+    flag, err = be_check(MECH, rho0, T, dt, np.maximum(rho0, 1e-20), constant_temperature=True)
+    _, w, dw = MECH.net_production_rates(np.maximum(rho0, 1e-20), T, T, jacobian=True)
+    est = np.abs(dt * w) / np.maximum(np.abs(1.0 - dt * np.diag(dw)), 1.0)
+    assert abs(err - est.max() / np.maximum(rho0, 1e-20).sum()) <= 1e-15 * err
+    # the bar is relative to each species' predicted change: 20 % off the
+    # change of O fails, 5 % passes
+    r0 = np.maximum(rho0, 1e-20)
+    for frac, flagged in ((1.2, True), (1.05, False)):
+        pred = good.copy()
+        pred[7] = r0[7] + frac * (good[7] - r0[7])
+        assert be_check(MECH, rho0, T, dt, pred, constant_temperature=True)[0] == flagged
