@@ -1,6 +1,7 @@
 """DG Euler and BR2 Navier-Stokes operators, restated from hybrid/spatial.py
-and hybrid/navier_stokes.py (numpy FP64).  WALE is off (dynamic viscosity
-only), as in every configuration this build runs."""
+and hybrid/navier_stokes.py (numpy FP64), with the optional WALE eddy
+viscosity of hybrid/solver.py:317-406 evaluated from the resolved polynomial
+gradient (the lifts excluded) as hybrid/navier_stokes.py:220-290 does."""
 
 import numpy as np
 
@@ -116,6 +117,11 @@ class NavierStokesOperator(EulerOperator):
         super().__init__(mesh, solver, degree)
         self.br2_stabilization = float(polynomial_br2_penalty(degree)
                                        if br2_stabilization is None else br2_stabilization)
+        # hybrid/navier_stokes.py:17-25: WALE filter width V^(1/3) / (p + 1)
+        self.filter_widths = {
+            b.element_type: self.geometry.element_metrics[bid].integration_weights.sum(1)
+            ** (1.0 / 3.0) / (degree + 1)
+            for bid, b in enumerate(mesh.element_blocks)}
 
     def _trace_data(self, states, group, boundary_state):
         """hybrid/navier_stokes.py:27-110."""
@@ -203,7 +209,8 @@ class NavierStokesOperator(EulerOperator):
             ref = self.geometry.reference_elements[t]
             m = self.geometry.element_metrics[bid]
             Uq = np.einsum("qi,eiv->eqv", ref.basis(ref.volume_points), states[t])
-            F = s.viscous_physical_flux(Uq, grads[t])
+            F = s.viscous_physical_flux(Uq, grads[t], self.filter_widths[t][:, None],
+                                        turbulent_gradient=raw[t])
             res[t] = -np.einsum("eqid,eqdv,eq->eiv", m.physical_basis_gradients, F,
                                 m.integration_weights)
         eta = self.br2_stabilization
@@ -214,13 +221,17 @@ class NavierStokesOperator(EulerOperator):
             og = self._physical_basis_gradients(d["ob"], d["ocells"], d["opoints"])
             raw_o = np.einsum("fim,fqid->fqmd", d["vo_nodes"], og)
             loc_o = np.einsum("fqi,fimd->fqmd", d["obasis"], d["olift"])
-            Fo = s.viscous_physical_flux(d["Uo"], raw_o + eta * loc_o)
+            Fo = s.viscous_physical_flux(d["Uo"], raw_o + eta * loc_o,
+                                         self.filter_widths[d["to"]][d["ocells"]][:, None],
+                                         turbulent_gradient=raw_o)
             if d["nb"] >= 0:
                 ng = self._physical_basis_gradients(d["nb"], d["ncells"],
                                                     fm.neighbor_reference_points[idx])
                 raw_n = np.einsum("fim,fqid->fqmd", d["vn_nodes"], ng)
                 loc_n = np.einsum("fqi,fimd->fqmd", d["nbasis"], d["nlift"])
-                Fn = s.viscous_physical_flux(d["Un"], raw_n + eta * loc_n)
+                Fn = s.viscous_physical_flux(d["Un"], raw_n + eta * loc_n,
+                                             self.filter_widths[d["tn"]][d["ncells"]][:, None],
+                                             turbulent_gradient=raw_n)
                 Ff = 0.5 * (Fo + Fn)
             else:
                 Ff = Fo
@@ -243,7 +254,7 @@ class NavierStokesOperator(EulerOperator):
     def rhs(self, states, boundary_state=None):
         """hybrid/navier_stokes.py:338-343."""
         e = self.inviscid_rhs(states, boundary_state)
-        if self.solver.dynamic_viscosity == 0.0:
+        if self.solver.dynamic_viscosity == 0.0 and not self.solver.use_wale:
             return e
         v = self.viscous_rhs(states, boundary_state)
         return {k: e[k] + v[k] for k in e}

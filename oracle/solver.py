@@ -7,13 +7,18 @@ class CompressibleSolver:
     """hybrid/solver.py:12-98 without the mesh: gamma, R, mu, Pr, species count."""
 
     def __init__(self, gamma=1.4, gas_constant=1.0, dynamic_viscosity=0.0,
-                 prandtl=0.72, num_species=0, inviscid_flux="hllc"):
+                 prandtl=0.72, num_species=0, inviscid_flux="hllc", use_wale=False,
+                 wale_constant=0.5, turbulent_prandtl=0.9):
         self.gamma = float(gamma)
         self.gas_constant = float(gas_constant)
         self.dynamic_viscosity = float(dynamic_viscosity)
         self.prandtl = float(prandtl)
         self.num_species = int(num_species)
         self.inviscid_flux = inviscid_flux
+        # hybrid/solver.py:30-47
+        self.use_wale = bool(use_wale)
+        self.wale_constant = float(wale_constant)
+        self.turbulent_prandtl = float(turbulent_prandtl)
         # hybrid/solver.py:48-53
         self.specific_heat_cp = self.gamma * self.gas_constant / (self.gamma - 1.0)
         self.thermal_conductivity = (self.dynamic_viscosity * self.specific_heat_cp
@@ -150,12 +155,46 @@ class CompressibleSolver:
         T = W[..., 4] / (np.maximum(W[..., 0], 1.0e-14) * self.gas_constant)
         return np.concatenate([W[..., 1:4], T[..., None]], -1)
 
-    def viscous_physical_flux(self, U, grad):
-        """hybrid/solver.py:357-406 without WALE: grad is (..., 4, 3)."""
+    def wale_dynamic_viscosity(self, U, grad, width):
+        """hybrid/solver.py:317-355: WALE eddy viscosity rho nu_t from the
+        velocity gradient grad[..., 0:3, :], with the Ducros sensor and the
+        clips of the reference."""
+        g = grad[..., 0:3, :]
+        S = 0.5 * (g + np.swapaxes(g, -1, -2))
+        g2 = np.einsum("...ik,...kj->...ij", g, g)
+        sym2 = 0.5 * (g2 + np.swapaxes(g2, -1, -2))
+        tr2 = np.trace(g2, axis1=-2, axis2=-1)
+        Sd = sym2 - tr2[..., None, None] * np.eye(3) / 3.0
+        snorm = np.sum(S ** 2, axis=(-2, -1))
+        div = g[..., 0, 0] + g[..., 1, 1] + g[..., 2, 2]
+        cx = g[..., 2, 1] - g[..., 1, 2]
+        cy = g[..., 0, 2] - g[..., 2, 0]
+        cz = g[..., 1, 0] - g[..., 0, 1]
+        vort = cx ** 2 + cy ** 2 + cz ** 2
+        ducros = vort / (vort + div ** 2 + 1.0e-12)
+        sdnorm = np.sum(Sd ** 2, axis=(-2, -1))
+        ss = np.clip(snorm, 0.0, 1.0e8)
+        st = np.clip(sdnorm, 0.0, 1.0e10)
+        with np.errstate(over="ignore", invalid="ignore"):
+            nu = ((self.wale_constant * width) ** 2 * st ** 1.5
+                  / (ss ** 2.5 + st ** 1.25 + 1.0e-30)) * ducros
+        nu = np.nan_to_num(np.clip(nu, 0.0, 50.0), nan=0.0, posinf=50.0)
+        return np.maximum(U[..., 0], 0.0) * nu
+
+    def viscous_physical_flux(self, U, grad, width=None, turbulent_gradient=None):
+        """hybrid/solver.py:357-406: grad is (..., 4, 3); with WALE and a
+        filter width the eddy viscosity (from turbulent_gradient, the
+        resolved polynomial gradient, when given) adds to mu and, over the
+        turbulent Prandtl number, to kappa."""
         W = self.conserved_to_primitive(U)
         vel = W[..., 1:4]
         mu = self.dynamic_viscosity
         kappa = self.thermal_conductivity
+        if self.use_wale and width is not None:
+            mt = self.wale_dynamic_viscosity(
+                U, grad if turbulent_gradient is None else turbulent_gradient, width)
+            mu = mu + mt[..., None, None]
+            kappa = kappa + mt * self.specific_heat_cp / self.turbulent_prandtl
         gu = grad[..., 0:3, :]
         gT = grad[..., 3, :]
         div = np.trace(gu, axis1=-2, axis2=-1)
@@ -165,7 +204,7 @@ class CompressibleSolver:
         work = np.einsum("...i,...id->...d", vel, tau)
         flux = np.zeros(U.shape[:-1] + (3, U.shape[-1]))
         flux[..., :, 1:4] = mom
-        flux[..., :, 4] = work + kappa * gT
+        flux[..., :, 4] = work + np.asarray(kappa)[..., None] * gT
         return flux
 
 

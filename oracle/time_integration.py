@@ -38,27 +38,9 @@ class SSPRK3:
         return out
 
     def stable_dt(self, states):
-        """hybrid/time_integration.py:93-169 (no WALE)."""
-        s = self.solver
-        p = self.operator.degree
-        k = 2 * p + 1
-        rates = []
-        for b in self.operator.mesh.element_blocks:
-            t = b.element_type
-            W = s.conserved_to_primitive(states[t])
-            rho = np.maximum(W[..., 0], 1e-14)
-            pr = np.maximum(W[..., 4], 1e-14)
-            speed = np.linalg.norm(W[..., 1:4], axis=-1) + np.sqrt(s.gamma * pr / rho)
-            L = self.characteristic_lengths[t]
-            rate = k * speed.max(1) / L
-            if s.dynamic_viscosity > 0.0:
-                rmin = np.maximum(rho.mean(1), 0.1)
-                diff = np.maximum((4.0 / 3.0) * s.dynamic_viscosity / rmin,
-                                  s.thermal_conductivity / (rmin * s.specific_heat_cp))
-                pen = float(getattr(self.operator, "br2_stabilization", 1.0))
-                rate = rate + 2.0 * pen * k ** 2 * diff / np.maximum(L, 0.01) ** 2
-            rate = np.nan_to_num(rate, nan=1e7, posinf=1e7)
-            rates.append(rate.max())
+        """hybrid/time_integration.py:93-169."""
+        rates = [np.nan_to_num(r, nan=1e7, posinf=1e7).max()
+                 for r in self._element_rates(states).values()]
         return float(np.clip(self.cfl / max(max(rates), 1e-12), 1e-7, 1e-3))
 
     def internal_connections(self):
@@ -81,8 +63,8 @@ class SSPRK3:
         return out
 
     def _element_rates(self, states):
-        """Per-element stability rate of hybrid/time_integration.py:93-169 (no
-        WALE), before the NaN replacement."""
+        """Per-element stability rate of hybrid/time_integration.py:93-169,
+        before the NaN replacement."""
         s = self.solver
         k = 2 * self.operator.degree + 1
         out = {}
@@ -94,10 +76,25 @@ class SSPRK3:
             speed = np.linalg.norm(W[..., 1:4], axis=-1) + np.sqrt(s.gamma * pr / rho)
             L = self.characteristic_lengths[t]
             rate = k * speed.max(1) / L
-            if s.dynamic_viscosity > 0.0:
+            if s.dynamic_viscosity > 0.0 or getattr(s, "use_wale", False):
                 rmin = np.maximum(rho.mean(1), 0.1)
-                diff = np.maximum((4.0 / 3.0) * s.dynamic_viscosity / rmin,
-                                  s.thermal_conductivity / (rmin * s.specific_heat_cp))
+                mu = np.full(rmin.shape, s.dynamic_viscosity)
+                kappa = np.full(rmin.shape, s.thermal_conductivity)
+                if getattr(s, "use_wale", False):
+                    # :119-148: the WALE viscosity of the resolved gradient,
+                    # maximum over the volume points
+                    bid = [b.element_type for b in self.operator.mesh.element_blocks].index(t)
+                    geo = self.operator.geometry
+                    ref = geo.reference_elements[t]
+                    m = geo.element_metrics[bid]
+                    v = s.viscous_variables(states[t])
+                    grad = np.einsum("eim,eqid->eqmd", v, m.physical_basis_gradients)
+                    Uq = np.einsum("qi,eiv->eqv", ref.basis(ref.volume_points), states[t])
+                    width = m.integration_weights.sum(1) ** (1.0 / 3.0) / (self.operator.degree + 1)
+                    turb = s.wale_dynamic_viscosity(Uq, grad, width[:, None]).max(1)
+                    mu = mu + turb
+                    kappa = kappa + turb * s.specific_heat_cp / s.turbulent_prandtl
+                diff = np.maximum((4.0 / 3.0) * mu / rmin, kappa / (rmin * s.specific_heat_cp))
                 pen = float(getattr(self.operator, "br2_stabilization", 1.0))
                 rate = rate + 2.0 * pen * k ** 2 * diff / np.maximum(L, 0.01) ** 2
             out[t] = rate
