@@ -41,7 +41,12 @@ extern "C" {
 typedef struct fsg_context fsg_context;
 
 /* Physics of HybridCompressibleSolver (hybrid/solver.py:20-53) and the BR2
- * penalty of HybridNavierStokesOperator (hybrid/navier_stokes.py:10-16). */
+ * penalty of HybridNavierStokesOperator (hybrid/navier_stokes.py:10-16).
+ * use_wale adds the WALE eddy viscosity of hybrid/solver.py:317-406 to mu
+ * (and mu_t c_p / Pr_t to kappa) in the viscous term, evaluated from the
+ * resolved polynomial gradient with each block's filter widths; it needs
+ * fsg_block_desc.width on every block and runs the vector element kernel
+ * and the dense face gradients on every block. */
 typedef struct {
   double gamma;
   double gas_constant;
@@ -52,6 +57,10 @@ typedef struct {
   int32_t nvars;         /* 5 + number of species (5 or 16) */
   int32_t degree;        /* 1, 2 or 3 */
   int32_t dtype;         /* FSG_F64 or FSG_F32 */
+  int32_t use_wale;      /* 0 or 1 */
+  int32_t reserved;
+  double wale_constant;     /* C_w (0.5 in the reference) */
+  double turbulent_prandtl; /* Pr_t (0.9 in the reference) */
 } fsg_params;
 
 /* One element block (hybrid/mesh.py:122-139 + hybrid/geometry.py:27-36 for
@@ -79,6 +88,9 @@ typedef struct {
   const void* minvT; /* (N, N) transposed reference inverse mass */
   const void* sf1d;  /* hexahedra at p >= 2: 1-D factors B1 (Q1, N1), D1 (Q1, N1),
                         M1^-1 (N1, N1) of the volume operators, or null */
+  const void* width; /* (E) WALE filter width V^(1/3) / (p + 1)
+                        (hybrid/navier_stokes.py:17-25); may be null
+                        without use_wale */
 } fsg_block_desc;
 
 /* One face group: (face type, owner block, neighbour block) as in

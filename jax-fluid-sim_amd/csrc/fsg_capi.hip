@@ -82,6 +82,9 @@ Phys<real> make_phys(const fsg_params& p, const fsg_boundary* b) {
   ph.kappa = real(p.thermal_conductivity);
   ph.eta = real(p.br2_penalty);
   ph.rusanov = p.inviscid_flux == 1;
+  ph.wale = p.use_wale != 0;
+  ph.wale_constant = real(p.wale_constant);
+  ph.cp_over_prt = real(p.gamma * p.gas_constant / (p.gamma - 1.0) / p.turbulent_prandtl);
   if (b) {
     ph.has_boundary = b->has_boundary;
     ph.has_wall_temperature = b->has_wall_temperature;
@@ -107,13 +110,15 @@ int run_faces(fsg_context* c, const void* const* U, int terms, const int32_t* co
     a.owner_sup = g.owner_sup;
     a.fw = static_cast<const real*>(g.face_weights);
     a.bflags = g.bflags;
-    if (c->hex_sum_fact != 0 && g.face_sf1d) {
+    // WALE needs the raw face gradient apart from the lift: dense gradients
+    if (c->hex_sum_fact != 0 && g.face_sf1d && !c->params.use_wale) {
       a.owner_sf = g.owner_sf;
       a.neighbor_sf = g.neighbor_sf;
       a.face_sf1d = static_cast<const real*>(g.face_sf1d);
     }
     a.jinv_o = static_cast<const real*>(bo.jinv);
     a.scale_o = static_cast<const real*>(bo.scale);
+    a.width_o = static_cast<const real*>(bo.width);
     a.out_o = fo + g.out_owner;
     if (mask) a.mask_o = mask[g.owner_block];
     int tn = -1;
@@ -127,6 +132,7 @@ int run_faces(fsg_context* c, const void* const* U, int terms, const int32_t* co
       a.neighbor_sup = g.neighbor_sup;
       a.jinv_n = static_cast<const real*>(bn.jinv);
       a.scale_n = static_cast<const real*>(bn.scale);
+      a.width_n = static_cast<const real*>(bn.width);
       a.out_n = fo + g.out_neighbor;
       if (mask) a.mask_n = mask[g.neighbor_block];
     }
@@ -159,7 +165,10 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.wq = static_cast<const real*>(d.wq);
     a.minv = static_cast<const real*>(d.minv);
     a.minvT = static_cast<const real*>(d.minvT);
-    a.sf1d = c->hex_sum_fact != 0 ? static_cast<const real*>(d.sf1d) : nullptr;
+    // WALE runs the vector element kernel (raw gradient apart from the lift)
+    const bool wale = c->params.use_wale != 0;
+    a.sf1d = c->hex_sum_fact != 0 && !wale ? static_cast<const real*>(d.sf1d) : nullptr;
+    a.width = static_cast<const real*>(d.width);
     a.jinv = static_cast<const real*>(d.jinv);
     a.scale = static_cast<const real*>(d.scale);
     a.detw = static_cast<const real*>(d.detw);
@@ -178,7 +187,7 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.diag = reinterpret_cast<StageDiag*>(diag);
     a.mask = mask ? mask[b] : nullptr;
     long ev = timing_begin(c, int(c->groups.size() + b), s, mask != nullptr);
-    const int mfma = c->element_mfma > 0 ? 1 : 0;
+    const int mfma = c->element_mfma > 0 && !wale ? 1 : 0;
     int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s, mfma);
     timing_end(c, ev, s);
     if (rc) return fail(rc, "element launch failed (type %d): %s", d.element_type,
@@ -237,6 +246,15 @@ int fsg_create(const fsg_params* params, int32_t nblocks, const fsg_block_desc* 
   for (int b = 0; b < nblocks; ++b)
     if (blocks[b].element_type < 0 || blocks[b].element_type > 2 || blocks[b].count < 0)
       return fail(-15, "block %d invalid", b);
+  if (p.use_wale != 0 && p.use_wale != 1) return fail(-18, "use_wale %d not 0 or 1", p.use_wale);
+  if (p.use_wale) {
+    if (!(p.wale_constant >= 0.0) || !(p.turbulent_prandtl > 0.0))
+      return fail(-18, "WALE constant %g / turbulent Prandtl number %g", p.wale_constant,
+                  p.turbulent_prandtl);
+    for (int b = 0; b < nblocks; ++b)
+      if (!blocks[b].width && blocks[b].count > 0)
+        return fail(-18, "WALE needs the filter widths of block %d", b);
+  }
   for (int g = 0; g < ngroups; ++g) {
     const fsg_group_desc& d = groups[g];
     if (d.owner_block < 0 || d.owner_block >= nblocks || d.neighbor_block >= nblocks ||

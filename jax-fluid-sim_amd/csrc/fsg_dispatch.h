@@ -91,7 +91,12 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
 
 #define FSG_FACE_CASE(FT, SUB, TO, TN, P, V, real)                                             \
   case ((FT * 2 + SUB) * 4 + TO) * 4 + (TN + 1):                                               \
-    hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real>), grid, block, 0, s, a, ph); \
+    if (ph.wale)                                                                                \
+      hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, true>), grid, block, 0, s, a, \
+                         ph);                                                                   \
+    else                                                                                        \
+      hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, false>), grid, block, 0, s,  \
+                         a, ph);                                                                \
     break;
 
 }  // namespace fsg

@@ -13,6 +13,7 @@
 
 #include "../../include/fluid_gfx950.h"
 #include "fsg_context.h"
+#include "fsg_physics.h"
 
 struct fsg_dt {
   fsg_context* ctx;
@@ -39,7 +40,76 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 struct RateParams {
   double gamma, mu, kappa, cp, pen, k;
+  int wale;            // add the WALE eddy viscosity (hybrid/time_integration.py:119-148)
+  double cw, cp_prt;   // C_w and c_p / Pr_t
 };
+
+// Per-block tables of the WALE term: basis and reference gradients at the
+// volume points, J^-1 and the filter widths (context dtype).
+template <typename real>
+struct WaleTables {
+  const real* Bt;     // (N, Q)
+  const real* Dt;     // (N, 3, Q)
+  const real* jinv;   // (E, 9)
+  const real* width;  // (E)
+  int Q;
+};
+
+// max over the volume points of the WALE viscosity of the resolved velocity
+// gradient (the physical basis gradients applied to the nodal velocities,
+// density interpolated to the point), in FP64.  One wave per element, at
+// most 64 nodes: lane i holds node i's density and velocity and every lane
+// reads them by shuffles for its points.
+template <typename real>
+__device__ __forceinline__ double wale_max(const real* U, int N, int V, int64_t e,
+                                           const WaleTables<real>& w, double cw, int lane) {
+  double rho = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  if (lane < N) {
+    const real* u = U + (int64_t)lane * V;
+    rho = double(u[0]);
+    const double safe = fmax(rho, 1.0e-14);
+    u0 = double(u[1]) / safe;
+    u1 = double(u[2]) / safe;
+    u2 = double(u[3]) / safe;
+  }
+  double J[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = double(w.jinv[e * 9 + k]);
+  const double width = double(w.width[e]);
+  double mx = 0.0;
+  for (int q0 = 0; q0 < w.Q; q0 += 64) {
+    const int q = q0 + lane;
+    const int qc = q < w.Q ? q : w.Q - 1;
+    double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, rq = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const double ri = __shfl(rho, i, 64);
+      const double vi[3] = {__shfl(u0, i, 64), __shfl(u1, i, 64), __shfl(u2, i, 64)};
+      const double b = double(w.Bt[(int64_t)i * w.Q + qc]);
+      const double d[3] = {double(w.Dt[((int64_t)i * 3 + 0) * w.Q + qc]),
+                           double(w.Dt[((int64_t)i * 3 + 1) * w.Q + qc]),
+                           double(w.Dt[((int64_t)i * 3 + 2) * w.Q + qc])};
+      rq += b * ri;
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) G[m * 3 + r] += d[r] * vi[m];
+    }
+    double g[9];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        g[m * 3 + c] = G[m * 3 + 0] * J[0 * 3 + c] + G[m * 3 + 1] * J[1 * 3 + c] +
+                       G[m * 3 + 2] * J[2 * 3 + c];
+    const double mt = fsg::wale_viscosity<double>(g, rq, width, cw);
+    if (q < w.Q) mx = mt != mt ? mt : (mx != mx ? mx : fmax(mx, mt));
+  }
+  // NaN-propagating wave maximum
+  bool nan = mx != mx;
+  nan = __any(nan);
+  mx = wave_max(nan ? 0.0 : mx);
+  return nan ? __longlong_as_double(0x7ff8000000000000LL) : mx;
+}
 
 // Stability rate of one element (one wave per element, lanes over nodes):
 // k max_i(|u_i| + c_i) / L, plus 2 eta k^2 max(4/3 mu, kappa / cp) / rho_mean
@@ -47,7 +117,8 @@ struct RateParams {
 // 1e-14 as in hybrid/time_integration.py:104-108.
 template <typename real>
 __device__ __forceinline__ double element_rate(const real* U, int N, int V, double L,
-                                               const RateParams& q, int lane) {
+                                               const RateParams& q, int lane, int64_t e,
+                                               const WaleTables<real>& w) {
   double smax = 0.0, rsum = 0.0;
   bool nan = false;
   for (int i = lane; i < N; i += 64) {
@@ -68,9 +139,15 @@ __device__ __forceinline__ double element_rate(const real* U, int N, int V, doub
   rsum = wave_sum(rsum);
   nan = __any(nan);
   double rate = q.k * smax / L;
-  if (q.mu > 0.0) {
+  if (q.mu > 0.0 || q.wale) {
     const double rmin = fmax(rsum / N, 0.1);
-    const double diff = fmax((4.0 / 3.0) * q.mu / rmin, q.kappa / (rmin * q.cp));
+    double mu = q.mu, kappa = q.kappa;
+    if (q.wale) {
+      const double mt = wale_max<real>(U, N, V, e, w, q.cw, lane);
+      mu += mt;
+      kappa += mt * q.cp_prt;
+    }
+    const double diff = fmax((4.0 / 3.0) * mu / rmin, kappa / (rmin * q.cp));
     const double lv = fmax(L, 0.01);
     rate = rate + 2.0 * q.pen * q.k * q.k * diff / (lv * lv);
   }
@@ -83,12 +160,12 @@ __device__ __forceinline__ double element_rate(const real* U, int N, int V, doub
 template <typename real>
 __global__ __launch_bounds__(WG) void max_rate_kernel(const real* __restrict__ U, int64_t E,
                                                       int N, int V, const double* __restrict__ L,
-                                                      RateParams q,
+                                                      RateParams q, WaleTables<real> w,
                                                       unsigned long long* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   const int64_t e = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
   if (e >= E) return;
-  double r = element_rate<real>(U + e * N * V, N, V, L[e], q, lane);
+  double r = element_rate<real>(U + e * N * V, N, V, L[e], q, lane, e, w);
   if (r != r || r > 1.0e300) r = 1.0e7;
   if (lane == 0) atomicMax(out, (unsigned long long)__double_as_longlong(r));
 }
@@ -116,12 +193,12 @@ __global__ void finish_dt_kernel(const unsigned long long* __restrict__ maxr, do
 template <typename real>
 __global__ __launch_bounds__(WG) void local_rate_kernel(const real* __restrict__ U, int64_t E,
                                                         int N, int V, const double* __restrict__ L,
-                                                        RateParams q, double cfl,
-                                                        double* __restrict__ dt) {
+                                                        RateParams q, WaleTables<real> w,
+                                                        double cfl, double* __restrict__ dt) {
   const int lane = threadIdx.x & 63;
   const int64_t e = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
   if (e >= E) return;
-  const double r = element_rate<real>(U + e * N * V, N, V, L[e], q, lane);
+  const double r = element_rate<real>(U + e * N * V, N, V, L[e], q, lane, e, w);
   if (lane == 0) dt[e] = r != r ? r : cfl / fmax(r, 1.0e-12);
 }
 
@@ -307,7 +384,26 @@ RateParams rate_params(const fsg_context* c) {
   q.cp = c->params.gamma * c->params.gas_constant / (c->params.gamma - 1.0);
   q.pen = c->params.br2_penalty;
   q.k = 2.0 * c->params.degree + 1.0;
+  q.wale = c->params.use_wale;
+  q.cw = c->params.wale_constant;
+  q.cp_prt = c->params.use_wale ? q.cp / c->params.turbulent_prandtl : 0.0;
   return q;
+}
+
+int vol_points_of(int type, int p) {
+  if (p == 1) return type == 0 ? 8 : (type == 1 ? 6 : 4);
+  return (p == 2 && type == 0) ? 27 : 125;
+}
+
+template <typename real>
+fsgdt::WaleTables<real> wale_tables(const fsg_context* c, const fsg_block_desc& bd) {
+  fsgdt::WaleTables<real> w;
+  w.Bt = static_cast<const real*>(bd.Bt);
+  w.Dt = static_cast<const real*>(bd.Dt);
+  w.jinv = static_cast<const real*>(bd.jinv);
+  w.width = static_cast<const real*>(bd.width);
+  w.Q = vol_points_of(bd.element_type, c->params.degree);
+  return w;
 }
 
 int nodes_of(int type, int p) {
@@ -378,11 +474,11 @@ int fsg_stable_dt(fsg_dt* d, const void* const* U, double cfl, double* dt_out,
     if (c->params.dtype == FSG_F64)
       hipLaunchKernelGGL(fsgdt::max_rate_kernel<double>, dim3(g), dim3(fsgdt::WG), 0, s,
                          static_cast<const double*>(U[b]), bd.count, N, V, d->blocks[b].length,
-                         q, mx);
+                         q, wale_tables<double>(c, bd), mx);
     else
       hipLaunchKernelGGL(fsgdt::max_rate_kernel<float>, dim3(g), dim3(fsgdt::WG), 0, s,
                          static_cast<const float*>(U[b]), bd.count, N, V, d->blocks[b].length,
-                         q, mx);
+                         q, wale_tables<float>(c, bd), mx);
   }
   hipLaunchKernelGGL(fsgdt::finish_dt_kernel, dim3(1), dim3(64), 0, s, mx, cfl, dt_out, state);
   return dt_launched();
@@ -411,11 +507,11 @@ int fsg_local_dt(fsg_dt* d, const void* const* U, double cfl, void* const* dt_el
     if (c->params.dtype == FSG_F64)
       hipLaunchKernelGGL(fsgdt::local_rate_kernel<double>, dim3(g), dim3(fsgdt::WG), 0, s,
                          static_cast<const double*>(U[b]), bd.count, N, V, d->blocks[b].length,
-                         q, cfl, dt[b]);
+                         q, wale_tables<double>(c, bd), cfl, dt[b]);
     else
       hipLaunchKernelGGL(fsgdt::local_rate_kernel<float>, dim3(g), dim3(fsgdt::WG), 0, s,
                          static_cast<const float*>(U[b]), bd.count, N, V, d->blocks[b].length,
-                         q, cfl, dt[b]);
+                         q, wale_tables<float>(c, bd), cfl, dt[b]);
   }
   for (int round = 0; round < 2; ++round)
     for (const fsg_dt_conn& k : d->conns) {

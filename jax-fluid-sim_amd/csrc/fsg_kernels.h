@@ -44,6 +44,7 @@ struct ElemArgs {
   const real* jinv;       // (E, 9)  [r*3+d] = d xi_r / d x_d
   const real* scale;      // (E) min(1, V/1e-4) / |det J|
   const real* detw;       // (E) |det J|
+  const real* width;      // (E) WALE filter width (read only with WALE on)
   const int64_t* rec_ptr; // (E+1)
   const int64_t* rec_off; // face-output offsets (reals)
   const int32_t* rec_sup; // offsets into the support-index array
@@ -80,6 +81,8 @@ struct FaceArgs {
   const real* scale_o;
   const real* jinv_n;
   const real* scale_n;
+  const real* width_o;     // WALE filter widths of the owner / neighbour block
+  const real* width_n;
   real* out_o;             // (F, FO, V+12)
   real* out_n;             // (F, FN, V+12)
   int64_t count;
@@ -369,14 +372,25 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
 #pragma unroll
         for (int c = 0; c < 12; ++c) g[c] += b * sL[i * 12 + c];
       }
+      real raw[12];
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-          g[m * 3 + d] += G[m * 3 + 0] * J[0 * 3 + d] + G[m * 3 + 1] * J[1 * 3 + d] +
-                          G[m * 3 + 2] * J[2 * 3 + d];
+        for (int d = 0; d < 3; ++d) {
+          raw[m * 3 + d] = G[m * 3 + 0] * J[0 * 3 + d] + G[m * 3 + 1] * J[1 * 3 + d] +
+                           G[m * 3 + 2] * J[2 * 3 + d];
+          g[m * 3 + d] += raw[m * 3 + d];
+        }
+      // WALE from the resolved gradient, the lift excluded
+      // (hybrid/navier_stokes.py:220-290)
+      real mu = ph.mu, kappa = ph.kappa;
+      if (ph.wale) {
+        const real mt = wale_viscosity<real>(raw, Up[0], a.width[e], ph.wale_constant);
+        mu += mt;
+        kappa += mt * ph.cp_over_prt;
+      }
       real Fv[3 * VF];
-      viscous_flux<VF, real>(Up, g, gamma, ph.mu, ph.kappa, Fv);
+      viscous_flux<VF, real>(Up, g, gamma, mu, kappa, Fv);
 #pragma unroll
       for (int k = 0; k < 3 * VF; ++k) dF[k] -= Fv[k];
     }
@@ -614,7 +628,10 @@ __device__ __forceinline__ void face_grad_hex_sf(const int32_t* __restrict__ met
   }
 }
 
-template <int FT, int TO, int TN, int SUB, int P, int V, typename real>
+// WALE: the variant that adds the WALE eddy viscosity of the resolved face
+// gradient (a separate instantiation, so that the plain kernels keep their
+// registers).
+template <int FT, int TO, int TN, int SUB, int P, int V, typename real, bool WALE>
 __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> ph) {
   constexpr int NO = n_nodes(TO, P);
   constexpr int NN = TN >= 0 ? n_nodes(TN, P) : 1;
@@ -649,6 +666,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   __shared__ real sUU[NU], sAA[NA], svo[NO * 4], svn[NN * 4];
   __shared__ real sSo[QF * V], sSn[QF * V], sPo[QF * V], sPn[QF * V], sJ[QF * 4];
   __shared__ real sbo[FO * 12], sbn[FN * 12];
+  __shared__ real sMt[WALE ? 2 * QF : 1]; // WALE eddy viscosity per side and point
   __shared__ real sB[2][6];              // anchor normal flux [5] and u.n per side
   real* const sUo = sUU;                 // [SO][V], slot 0 = node 0, slot 1 + k = support node k
   real* const sUn = sUU + SO * V;        // [SN][V]
@@ -978,7 +996,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     // support nodes (the gradient table is read once per (point, node)); FP64
     // dense keeps one lane per (point, variable), which needs fewer registers.
     bool sf_o = false, sf_n = false;
-    if constexpr (SFO || SFN) {
+    if constexpr ((SFO || SFN) && !WALE) {
       if (a.face_sf1d) {
         constexpr int MW = 4 + 2 * QF;
         if constexpr (SFO) {
@@ -1007,7 +1025,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     if constexpr (sizeof(real) == 4) {
       static_assert(QF <= 16 || QF == 25, "face points per wave");
       auto side = [&](const real* tb, const real* sv, const real* sbb, const real* Jx, real sc,
-                      int NE, int NF, real* sg) {
+                      int NE, int NF, real* sg, const real* sP, real wd, real* smt) {
         for (int q0 = 0; q0 < QF; q0 += 16) {
           const int q = q0 + (tid >> 2), part = tid & 3;
           real G[12], L[12];
@@ -1042,27 +1060,35 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
             L[k] += __shfl_xor(L[k], 1, 64);
             L[k] += __shfl_xor(L[k], 2, 64);
           }
-          if (q < QF && part == 0)
+          if (q < QF && part == 0) {
+            real raw[12];
 #pragma unroll
             for (int m = 0; m < 4; ++m)
 #pragma unroll
-              for (int d = 0; d < 3; ++d)
-                sg[q * 12 + m * 3 + d] = G[m * 3 + 0] * Jx[0 * 3 + d] + G[m * 3 + 1] * Jx[1 * 3 + d] +
-                                         G[m * 3 + 2] * Jx[2 * 3 + d] + ph.eta * (sc * L[m * 3 + d]);
+              for (int d = 0; d < 3; ++d) {
+                raw[m * 3 + d] = G[m * 3 + 0] * Jx[0 * 3 + d] + G[m * 3 + 1] * Jx[1 * 3 + d] +
+                                 G[m * 3 + 2] * Jx[2 * 3 + d];
+                sg[q * 12 + m * 3 + d] = raw[m * 3 + d] + ph.eta * (sc * L[m * 3 + d]);
+              }
+            // WALE from the resolved gradient, the lift excluded
+            if constexpr (WALE) smt[q] = wale_viscosity<real>(raw, sP[q * V], wd, ph.wale_constant);
+          }
         }
       };
       if (!sf_o) {
         real Jo[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) Jo[k] = a.jinv_o[eo * 9 + k];
-        side(tbo, svo, sbo, Jo, a.scale_o[eo], NO, FO, sgo);
+        side(tbo, svo, sbo, Jo, a.scale_o[eo], NO, FO, sgo, sPo,
+             WALE ? a.width_o[eo] : real(0), sMt);
       }
       if constexpr (NBR) {
         if (!sf_n) {
           real Jn[9];
 #pragma unroll
           for (int k = 0; k < 9; ++k) Jn[k] = a.jinv_n[en * 9 + k];
-          side(tbn, svn, sbn, Jn, a.scale_n[en], NN, FN, sgn);
+          side(tbn, svn, sbn, Jn, a.scale_n[en], NN, FN, sgn, sPn,
+               WALE ? a.width_n[en] : real(0), WALE ? sMt + QF : sMt);
         }
       }
     } else {
@@ -1088,10 +1114,22 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
 #pragma unroll
           for (int d = 0; d < 3; ++d) L[d] += c * sbo[kk * 12 + m * 3 + d];
         }
+        real rd[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-          sgo[q * 12 + m * 3 + d] = G0 * Jo[0 * 3 + d] + G1 * Jo[1 * 3 + d] + G2 * Jo[2 * 3 + d] +
-                                    ph.eta * (so * L[d]);
+        for (int d = 0; d < 3; ++d) {
+          rd[d] = G0 * Jo[0 * 3 + d] + G1 * Jo[1 * 3 + d] + G2 * Jo[2 * 3 + d];
+          sgo[q * 12 + m * 3 + d] = rd[d] + ph.eta * (so * L[d]);
+        }
+        // WALE: the point's four lanes (one wave, same iteration) hand the
+        // raw velocity gradient rows to the m = 0 lane
+        if constexpr (WALE) {
+          real r9[9];
+#pragma unroll
+          for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) r9[mm * 3 + d] = __shfl(rd[d], (tid & ~3) + mm, 64);
+          if (m == 0) sMt[q] = wale_viscosity<real>(r9, sPo[q * V], a.width_o[eo], ph.wale_constant);
+        }
       }
       if constexpr (NBR) {
         const real sn = a.scale_n[en];
@@ -1116,10 +1154,21 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
 #pragma unroll
             for (int d = 0; d < 3; ++d) L[d] += c * sbn[kk * 12 + m * 3 + d];
           }
+          real rd[3];
 #pragma unroll
-          for (int d = 0; d < 3; ++d)
-            sgn[q * 12 + m * 3 + d] = G0 * Jn[0 * 3 + d] + G1 * Jn[1 * 3 + d] +
-                                      G2 * Jn[2 * 3 + d] + ph.eta * (sn * L[d]);
+          for (int d = 0; d < 3; ++d) {
+            rd[d] = G0 * Jn[0 * 3 + d] + G1 * Jn[1 * 3 + d] + G2 * Jn[2 * 3 + d];
+            sgn[q * 12 + m * 3 + d] = rd[d] + ph.eta * (sn * L[d]);
+          }
+          if constexpr (WALE) {
+            real r9[9];
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) r9[mm * 3 + d] = __shfl(rd[d], (tid & ~3) + mm, 64);
+            if (m == 0)
+              sMt[QF + q] = wale_viscosity<real>(r9, sPn[q * V], a.width_n[en], ph.wale_constant);
+          }
         }
       }
     }
@@ -1137,9 +1186,16 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
       real Fx[3 * V];
 #pragma unroll
       for (int k = 0; k < 3 * V; ++k) Fx[k] = real(0);
-      if (q < QF)
-        viscous_flux<V, real>((side ? sPn : sPo) + q * V, (side ? sgn : sgo) + q * 12, gamma,
-                              ph.mu, ph.kappa, Fx);
+      if (q < QF) {
+        real mu = ph.mu, kappa = ph.kappa;
+        if constexpr (WALE) {
+          const real mt = sMt[side * QF + q];
+          mu += mt;
+          kappa += mt * ph.cp_over_prt;
+        }
+        viscous_flux<V, real>((side ? sPn : sPo) + q * V, (side ? sgn : sgo) + q * 12, gamma, mu,
+                              kappa, Fx);
+      }
       real Fo[3][5];
 #pragma unroll
       for (int d = 0; d < 3; ++d)

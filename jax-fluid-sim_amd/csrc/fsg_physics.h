@@ -46,6 +46,9 @@ struct Phys {
   int has_wall_temperature;
   real wall_temperature;
   real farfield[16];     // primitive farfield [rho, u, v, w, p, Y...]
+  int wale;              // WALE eddy viscosity on (hybrid/solver.py:317-406)
+  real wale_constant;    // C_w
+  real cp_over_prt;      // c_p / Pr_t: eddy conductivity per eddy viscosity
 };
 
 template <typename real> __device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : b; }
@@ -352,7 +355,51 @@ __device__ __forceinline__ void viscous_variables(const real* U, real gamma, rea
   v[3] = w.p / (rmax(w.rho, real(1.0e-14)) * R);
 }
 
-// hybrid/solver.py:357-406 without WALE: grad[m*3+d] = d v_m / d x_d.
+// hybrid/solver.py:317-355: WALE eddy dynamic viscosity rho nu_t from the
+// resolved velocity gradient g[i*3+d] = d u_i / d x_d, the filter width and
+// the density.  With s = |S|^2 and t = |S^d|^2 (Frobenius norms of the
+// strain rate and of the traceless symmetric part of g^2), s clipped to
+// [0, 1e8] and t to [0, 1e10]: nu_t = (C_w D)^2 t^(3/2) / (s^(5/2) +
+// t^(5/4) + 1e-30) times the Ducros sensor |w|^2 / (|w|^2 + (div u)^2 +
+// 1e-12), clipped to [0, 50] (NaN to 0); rho floored at 0 (NaN kept).
+template <typename real>
+__device__ __forceinline__ real wale_viscosity(const real* g, real rho, real width, real cw) {
+  real g2[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      g2[i * 3 + j] = g[i * 3 + 0] * g[0 * 3 + j] + g[i * 3 + 1] * g[1 * 3 + j] +
+                      g[i * 3 + 2] * g[2 * 3 + j];
+  const real third = (g2[0] + g2[4] + g2[8]) / real(3);
+  real sn = real(0), sd = real(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const real sij = real(0.5) * (g[i * 3 + j] + g[j * 3 + i]);
+      const real dij = real(0.5) * (g2[i * 3 + j] + g2[j * 3 + i]) - (i == j ? third : real(0));
+      sn += sij * sij;
+      sd += dij * dij;
+    }
+  const real div = g[0] + g[4] + g[8];
+  const real cx = g[7] - g[5], cy = g[2] - g[6], cz = g[3] - g[1];
+  const real vort = cx * cx + cy * cy + cz * cz;
+  const real ducros = vort / (vort + div * div + real(1.0e-12));
+  // clips that keep NaN (both comparisons false)
+  const real ss = sn < real(0) ? real(0) : (sn > real(1.0e8) ? real(1.0e8) : sn);
+  const real st = sd < real(0) ? real(0) : (sd > real(1.0e10) ? real(1.0e10) : sd);
+  // t^(3/2) = t sqrt(t), t^(5/4) = t sqrt(sqrt(t)), s^(5/2) = s^2 sqrt(s)
+  const real rs = sqrt(ss), rt = sqrt(st);
+  const real cwd = cw * width;
+  real nu = (cwd * cwd * (st * rt) / (ss * ss * rs + st * sqrt(rt) + real(1.0e-30))) * ducros;
+  nu = nu > real(50) ? real(50) : (nu >= real(0) ? nu : real(0));
+  const real r = rho < real(0) ? real(0) : rho;
+  return r * nu;
+}
+
+// hybrid/solver.py:357-406: grad[m*3+d] = d v_m / d x_d; mu and kappa
+// include the eddy viscosity and conductivity when WALE is on.
 // Writes Fv[d*V + v]; only the momentum and energy columns are non-zero.
 template <int V, typename real>
 __device__ __forceinline__ void viscous_flux(const real* U, const real* grad, real gamma, real mu,

@@ -22,7 +22,9 @@ class Params(ctypes.Structure):
                 ("dynamic_viscosity", ctypes.c_double),
                 ("thermal_conductivity", ctypes.c_double), ("br2_penalty", ctypes.c_double),
                 ("inviscid_flux", ctypes.c_int32), ("nvars", ctypes.c_int32),
-                ("degree", ctypes.c_int32), ("dtype", ctypes.c_int32)]
+                ("degree", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("use_wale", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("wale_constant", ctypes.c_double), ("turbulent_prandtl", ctypes.c_double)]
 
 
 class BlockDesc(ctypes.Structure):
@@ -30,7 +32,7 @@ class BlockDesc(ctypes.Structure):
                 ("count", ctypes.c_int64)] + [
         (k, ctypes.c_void_p) for k in ("Bt", "Dt", "D", "wq", "minv", "jinv", "scale", "detw",
                                        "rec_ptr", "rec_off", "rec_sup", "rec_nf", "minvT",
-                                       "sf1d")]
+                                       "sf1d", "width")]
 
 
 class GroupDesc(ctypes.Structure):

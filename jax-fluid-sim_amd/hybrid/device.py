@@ -146,7 +146,9 @@ class DeviceMesh:
                 D=self._t(R["D"].transpose(0, 2, 1)), wq=self._t(R["weights"]),
                 minv=self._t(R["Minv"]), minvT=self._t(R["Minv"].T),
                 jinv=self._t(b.jinv.reshape(-1, 9)),
-                scale=self._t(scale), detw=self._t(np.abs(b.det)), volume=vol)
+                scale=self._t(scale), detw=self._t(np.abs(b.det)), volume=vol,
+                # WALE filter width V^(1/3) / (p + 1) (hybrid/navier_stokes.py:17-25)
+                width=self._t(vol ** (1.0 / 3.0) / (self.p + 1)))
             sf = hex_factors(b.element_type, self.p, R)
             t["sf1d"] = None if sf is None else self._t(sf)
             self.block_tables.append(t)
@@ -280,7 +282,7 @@ class DeviceMesh:
             d.element_type = TYPE_ID[t["type"]]
             d.count = t["count"]
             for k in ("Bt", "Dt", "D", "wq", "minv", "minvT", "jinv", "scale", "detw", "rec_ptr",
-                      "rec_off", "rec_sup", "rec_nf"):
+                      "rec_off", "rec_sup", "rec_nf", "width"):
                 setattr(d, k, t[k].data_ptr())
             d.sf1d = 0 if t["sf1d"] is None else t["sf1d"].data_ptr()
         groups = (nat.GroupDesc * max(len(self.groups), 1))()

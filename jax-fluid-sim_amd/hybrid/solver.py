@@ -44,15 +44,14 @@ class CompressibleStateLayout:
 class HybridCompressibleSolver:
     def __init__(self, mesh, species_names=(), gamma=1.4, gas_constant=1.0,
                  polynomial_degree=2, inviscid_flux="hllc", dynamic_viscosity=0.0,
-                 prandtl=0.72, use_wale=False, dtype=torch.float64, device="cuda"):
+                 prandtl=0.72, use_wale=False, wale_constant=0.5, turbulent_prandtl=0.9,
+                 dtype=torch.float64, device="cuda"):
         if isinstance(mesh, MeshArrays):
             mesh = Topology(mesh)
         if not isinstance(mesh, Topology):
             raise TypeError("mesh must be MeshArrays or Topology (Gmsh reading is not built)")
         if inviscid_flux not in ("hllc", "rusanov"):
             raise ValueError("inviscid_flux must be 'hllc' or 'rusanov'")
-        if use_wale:
-            raise NotImplementedError("WALE is not part of the device path")
         self.mesh = mesh
         self.layout = CompressibleStateLayout(species_names)
         self.gamma = float(gamma)
@@ -61,7 +60,10 @@ class HybridCompressibleSolver:
         self.inviscid_flux = inviscid_flux
         self.dynamic_viscosity = float(dynamic_viscosity)
         self.prandtl = float(prandtl)
-        self.use_wale = False
+        # hybrid/solver.py:30-47: WALE eddy viscosity in the viscous term
+        self.use_wale = bool(use_wale)
+        self.wale_constant = float(wale_constant)
+        self.turbulent_prandtl = float(turbulent_prandtl)
         # hybrid/solver.py:48-53
         self.specific_heat_cp = self.gamma * self.gas_constant / (self.gamma - 1.0)
         self.thermal_conductivity = (self.dynamic_viscosity * self.specific_heat_cp

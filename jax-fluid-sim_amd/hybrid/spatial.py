@@ -37,6 +37,9 @@ class HybridEulerOperator:
         p.nvars = solver.layout.number_of_variables
         p.degree = solver.polynomial_degree
         p.dtype = nat.F64 if solver.dtype == torch.float64 else nat.F32
+        p.use_wale = int(getattr(solver, "use_wale", False))
+        p.wale_constant = getattr(solver, "wale_constant", 0.5)
+        p.turbulent_prandtl = getattr(solver, "turbulent_prandtl", 0.9)
         self.device_mesh = DeviceMesh(solver.mesh, solver.polynomial_degree, p.nvars, p,
                                       solver.dtype, solver.device)
         self.block_types = [b.element_type for b in solver.mesh.blocks]
@@ -93,7 +96,8 @@ class HybridEulerOperator:
 
 
 class HybridNavierStokesOperator(HybridEulerOperator):
-    """hybrid/navier_stokes.py:9-347 (BR2, no WALE)."""
+    """hybrid/navier_stokes.py:9-347: BR2 viscous term, with the WALE eddy
+    viscosity of the resolved gradient when the solver has use_wale."""
 
     def viscous_residual(self, states, boundary_state=None):
         return self._call(states, nat.MODE_RESIDUAL, nat.TERM_VISCOUS, boundary_state)
@@ -105,9 +109,9 @@ class HybridNavierStokesOperator(HybridEulerOperator):
         return HybridEulerOperator.rhs(self, states, boundary_state)
 
     def rhs_terms(self):
-        """Terms of rhs(): the viscous part is skipped when mu == 0
-        (hybrid/navier_stokes.py:340-341)."""
-        if self.solver.dynamic_viscosity == 0.0:
+        """Terms of rhs(): the viscous part is skipped when mu == 0 and WALE
+        is off (hybrid/navier_stokes.py:340-341)."""
+        if self.solver.dynamic_viscosity == 0.0 and not self.solver.use_wale:
             return nat.TERM_EULER
         return nat.TERM_EULER | nat.TERM_VISCOUS
 

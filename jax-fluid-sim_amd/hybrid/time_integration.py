@@ -141,7 +141,8 @@ class HybridSSPRK3:
             pass
 
     def stable_dt(self, states, boundary_state=None):
-        """hybrid/time_integration.py:93-169 (no WALE): one wave per element
+        """hybrid/time_integration.py:93-169 (with the WALE viscosity of the
+        resolved gradient when the solver has use_wale): one wave per element
         forms its rate and an atomic maximum reduces them on the device;
         returns a 0-d FP64 tensor clipped to [1e-7, 1e-3] (no host
         synchronisation)."""
@@ -285,7 +286,8 @@ class HybridSSPRK3:
         op = self.operator
         stream = nat.stream_handle(self.solver.device)
         self._diag.reset(stream)
-        viscous = hasattr(op, "inviscid_rhs") and self.solver.dynamic_viscosity > 0.0
+        has_visc = self.solver.dynamic_viscosity > 0.0 or self.solver.use_wale
+        viscous = hasattr(op, "inviscid_rhs") and has_visc
         if viscous and self.fused_mood:
             total = op.rhs(curr, boundary_state)
             visc = {t: torch.empty_like(curr[t]) for t in op.block_types}
@@ -295,9 +297,8 @@ class HybridSSPRK3:
             return
         if hasattr(op, "inviscid_rhs"):
             inv = op.inviscid_rhs(curr, boundary_state)
-            # a zero viscosity gives an exactly zero viscous rhs: skip it
-            visc = (op.viscous_rhs(curr, boundary_state) if self.solver.dynamic_viscosity > 0.0
-                    else None)
+            # a zero viscosity without WALE gives an exactly zero viscous rhs: skip it
+            visc = op.viscous_rhs(curr, boundary_state) if has_visc else None
         else:
             inv, visc = op.rhs(curr, boundary_state), None
         work = {t: torch.empty_like(curr[t]) for t in op.block_types}

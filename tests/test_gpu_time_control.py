@@ -29,7 +29,7 @@ NAMES = {"xmin": "inlet", "xmax": "outlet", "ymin": "walls", "ymax": "walls",
          "zmin": "walls", "zmax": "walls"}
 
 
-def _case(name, dtype=torch.float64, mood=False, cfl=0.3, pfloor=1e-7):
+def _case(name, dtype=torch.float64, mood=False, cfl=0.3, pfloor=1e-7, wale=False):
     if name == "split":
         m = box_mesh(4, split_fraction=0.35, seed=7, plane_names=PLANE_NAMES)
         mu = 0.01
@@ -48,9 +48,9 @@ def _case(name, dtype=torch.float64, mood=False, cfl=0.3, pfloor=1e-7):
     else:
         raise KeyError(name)
     ds = HybridCompressibleSolver(m, polynomial_degree=2, dynamic_viscosity=mu, prandtl=0.71,
-                                  dtype=dtype)
+                                  use_wale=wale, dtype=dtype)
     dop = HybridNavierStokesOperator(ds)
-    os_ = CompressibleSolver(dynamic_viscosity=mu, prandtl=0.71)
+    os_ = CompressibleSolver(dynamic_viscosity=mu, prandtl=0.71, use_wale=wale)
     oop = NavierStokesOperator(from_arrays(m), os_, 2)
     db = FarfieldWallBoundary(ds, far)
     ob = OracleBoundary(os_, far, inlet_tags=sorted(db.inlet_tags),
@@ -86,6 +86,32 @@ def test_stable_and_local_dt(name):
     for t in wl:
         assert gl[t].shape == wl[t].shape
         assert np.abs(gl[t] - wl[t]).max() <= 1e-13 * wl[t].max(), t
+
+
+@pytest.mark.parametrize("name", ["split", "sliver"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_dt_with_wale(name, dtype):
+    """The CFL and element-local dt with the WALE viscosity of the resolved
+    gradient (hybrid/time_integration.py:119-148) in the viscous rate; the
+    WALE term moves the oracle's dt, so the comparison sees it.  FP64 states:
+    1e-13; FP32 states: the FP64 rate of the FP32 state against the oracle on
+    the same rounded state, 1e-7 (the basis tables and metrics rounded to
+    FP32, through the power laws of the model; measured 3e-9)."""
+    print(f"case {name} dt WALE {dtype}", flush=True)
+    ds, drk, db, ork, ob, st = _case(name, dtype=dtype, wale=True)
+    _, _, _, ork0, _, _ = _case(name)
+    U = ds.to_device(st)
+    st = _np(U)
+    want = ork.stable_dt(st)
+    assert abs(ork0.stable_dt(st) - want) > 1e-3 * want
+    tol = 1e-13 if dtype == torch.float64 else 1e-7
+    got = float(drk.stable_dt(U))
+    print(f"dt {got!r} oracle {want!r} rel {abs(got - want) / want:.2e}", flush=True)
+    assert abs(got - want) <= tol * want, (got, want)
+    gl = _np(drk.local_dt(U))
+    wl = ork.local_dt(st)
+    for t in wl:
+        assert np.abs(gl[t] - wl[t]).max() <= tol * wl[t].max(), t
 
 
 def test_conserved_totals_and_conservation():
