@@ -6,6 +6,7 @@ kernels of libfluid_gfx950.so.
 """
 
 from .boundary import FarfieldWallBoundary
+from .gmsh import read_gmsh, write_gmsh
 from .meshgen import box_mesh, smoke_mesh
 from .mood import (HybridModalShockLimiter, HybridMOODSubcellLimiter, HybridPositivityLimiter,
                    HybridTroubledCellSensor)
@@ -14,7 +15,7 @@ from .spatial import HybridEulerOperator, HybridNavierStokesOperator
 from .time_integration import HybridSSPRK3
 from .topology import Topology
 
-__all__ = ["FarfieldWallBoundary", "box_mesh", "smoke_mesh", "CompressibleStateLayout",
-           "HybridCompressibleSolver", "HybridEulerOperator", "HybridNavierStokesOperator",
+__all__ = ["FarfieldWallBoundary", "box_mesh", "smoke_mesh", "read_gmsh", "write_gmsh",
+           "CompressibleStateLayout", "HybridCompressibleSolver", "HybridEulerOperator", "HybridNavierStokesOperator",
            "HybridSSPRK3", "Topology", "HybridMOODSubcellLimiter", "HybridPositivityLimiter",
            "HybridTroubledCellSensor", "HybridModalShockLimiter"]

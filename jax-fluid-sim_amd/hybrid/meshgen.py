@@ -44,6 +44,7 @@ class MeshArrays:
     lengths: np.ndarray = None
     origin: np.ndarray = None
     default_tag: int = -1
+    face_tags: dict = None     # sorted corner ids -> tag (meshes read from files)
 
     @property
     def number_of_cells(self):
@@ -51,7 +52,18 @@ class MeshArrays:
 
     def tag_of(self, face_nodes):
         """Boundary tag of a face given its corner node ids (any shape
-        (..., k)): ``default_tag`` when set, else the box plane tag."""
+        (..., k)): the tag of the file's boundary face with those corners
+        (a boundary face without one raises, hybrid/mesh.py:477-483),
+        ``default_tag`` when set, else the box plane tag."""
+        if self.face_tags is not None:
+            nodes = np.asarray(face_nodes)
+            flat = nodes.reshape(-1, nodes.shape[-1])
+            out = np.array([self.face_tags.get(tuple(sorted(int(v) for v in row)), -1)
+                            for row in flat], np.int32).reshape(nodes.shape[:-1])
+            missing = int(np.count_nonzero(out < 0))
+            if missing:
+                raise ValueError(f"{missing} boundary faces lack physical tags")
+            return out
         if self.default_tag >= 0:
             return np.full(np.shape(face_nodes)[:-1], self.default_tag, np.int32)
         return plane_tags(self.points, face_nodes, self.lengths, self.origin)

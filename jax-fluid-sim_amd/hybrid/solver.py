@@ -6,10 +6,13 @@ hybrid/state.py:7-37.  The pointwise flux functions live in the HIP kernels;
 the helpers here only build states.
 """
 
+import os
+
 import numpy as np
 import torch
 
 from .elements import reference_tables
+from .gmsh import read_gmsh
 from .meshgen import MeshArrays
 from .topology import Topology
 
@@ -46,10 +49,13 @@ class HybridCompressibleSolver:
                  polynomial_degree=2, inviscid_flux="hllc", dynamic_viscosity=0.0,
                  prandtl=0.72, use_wale=False, wale_constant=0.5, turbulent_prandtl=0.9,
                  dtype=torch.float64, device="cuda"):
+        if isinstance(mesh, (str, os.PathLike)):
+            # the reference's mesh_path argument (hybrid/solver.py:20-35)
+            mesh = read_gmsh(mesh)
         if isinstance(mesh, MeshArrays):
             mesh = Topology(mesh)
         if not isinstance(mesh, Topology):
-            raise TypeError("mesh must be MeshArrays or Topology (Gmsh reading is not built)")
+            raise TypeError("mesh must be a Gmsh file path, MeshArrays or Topology")
         if inviscid_flux not in ("hllc", "rusanov"):
             raise ValueError("inviscid_flux must be 'hllc' or 'rusanov'")
         self.mesh = mesh

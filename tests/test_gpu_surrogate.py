@@ -135,6 +135,11 @@ def test_projection_conserves_elements_charge_and_mass(integrator):
 
 
 def test_out_of_hull_cells_fall_back_to_the_integrator(integrator):
+    """Cells outside the training hull fall back: with the integrator-judged
+    error check off, exactly the hull-flagged cells take the integrator's
+    result and the others the network's; with it on (the default), the
+    fallback set still holds every hull-flagged cell and every fallback cell
+    carries the integrator's result."""
     sur, rho, T, _ = _fitted(integrator, n=512)
     T2 = T.clone()
     T2[::7] = 60000.0                           # far outside the 3000-12000 K training hull
@@ -142,13 +147,25 @@ def test_out_of_hull_cells_fall_back_to_the_integrator(integrator):
     fb = fb.cpu().numpy()
     assert fb[::7].all()
     assert np.array_equal(out.cpu().numpy()[::7], rho.cpu().numpy()[::7])
-    full, _, fb2 = sur.step(rho, T2, 1e-7)
+    check = sur.check_rtol
+    sur.check_rtol = None
+    try:
+        full, _, fb2 = sur.step(rho, T2, 1e-7)
+    finally:
+        sur.check_rtol = check
     ref = integrator.step(rho[::7], T2[::7], 1e-7,
                           constant_temperature=sur.constant_temperature)[0]
     assert torch.equal(full[::7], ref)
     assert torch.equal(fb2.cpu(), torch.as_tensor(fb))
     keep = ~fb2
     assert torch.equal(full[keep], out[keep])
+    full, _, fb3 = sur.step(rho, T2, 1e-7)
+    fb3 = fb3.cpu()
+    assert bool(fb3[torch.as_tensor(fb)].all())
+    sel = fb3.nonzero().flatten().to(rho.device)
+    ref = integrator.step(rho[sel], T2[sel], 1e-7,
+                          constant_temperature=sur.constant_temperature)[0]
+    assert torch.equal(full[sel], ref)
 
 
 def test_neutral_cells_pass_the_projection_check(integrator):
