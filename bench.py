@@ -325,7 +325,8 @@ def roofline_of(row, prec, traffic=None):
 
 
 def reacting_rate(n=86, split=0.2, steps=20, warmup=2, error_steps=100, seed=1,
-                  train_cells=1 << 15, train_traj=24, train_steps=3000):
+                  train_cells=1 << 15, train_traj=24, train_steps=3000,
+                  paths=("f64_plain", "f32_surrogate")):
     """Reacting hybrid-mesh flow: the shock-bubble mesh family scaled up (unit
     box of n^3 hexahedra, a seeded 20 % split into wedges / tetrahedra; n = 86
     gives about 1.0M elements), p=2, 5 + 11 variables, Mach 3 air at 1500 K
@@ -339,7 +340,8 @@ def reacting_rate(n=86, split=0.2, steps=20, warmup=2, error_steps=100, seed=1,
     flow's own cells.  Same dt for both.  A timed step is the RK step and the
     chemistry; `steps` steps are timed after `warmup` untimed ones, both
     paths then continue untimed to `error_steps` steps for the L2 error of
-    the fast path against the plain FP64 path."""
+    the fast path against the plain FP64 path.  ``paths`` may name one path
+    only (profiling passes): then no comparison is made."""
     import torch
     from hybrid import (FarfieldWallBoundary, HybridCompressibleSolver, HybridMOODSubcellLimiter,
                         HybridNavierStokesOperator, HybridPositivityLimiter, HybridSSPRK3,
@@ -364,6 +366,9 @@ def reacting_rate(n=86, split=0.2, steps=20, warmup=2, error_steps=100, seed=1,
     error_steps = max(error_steps, steps)
     for key, dtype, use_sur in (("f64_plain", torch.float64, False),
                                 ("f32_surrogate", torch.float32, True)):
+        if key not in paths:
+            continue
+        print(f"[bench] reacting {key}: setting up", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         solver = HybridCompressibleSolver(mesh, species_names=SPECIES_NAMES, polynomial_degree=2,
                                           gamma=GAMMA, dynamic_viscosity=far[1] / 1.0e4,
@@ -478,6 +483,9 @@ def reacting_rate(n=86, split=0.2, steps=20, warmup=2, error_steps=100, seed=1,
         torch.cuda.empty_cache()
     for key in res:
         out[key] = res[key][0]
+    out["steps"], out["warmup"], out["error_steps"], out["dt"] = steps, warmup, error_steps, dt
+    if len(res) < 2:
+        return out
     labels = ["rho", "rho_u", "rho_v", "rho_w", "rho_E"] + [f"rho_{s}" for s in SPECIES_NAMES]
 
     def errors(a, b):
@@ -505,7 +513,6 @@ def reacting_rate(n=86, split=0.2, steps=20, warmup=2, error_steps=100, seed=1,
         errors(res["f32_surrogate"][2], res["f64_plain"][2])
     out["speedup_fast_vs_plain"] = (out["f64_plain"]["ms_per_step"]
                                     / out["f32_surrogate"]["ms_per_step"])
-    out["steps"], out["warmup"], out["error_steps"], out["dt"] = steps, warmup, error_steps, dt
     return out
 
 
@@ -788,6 +795,10 @@ def main():
     ap.add_argument("--only", choices=["tgv", "mood", "chem", "reacting"], default=None,
                     help="run one section only and print its JSON (for profiling)")
     ap.add_argument("--mood-steps", type=int, default=5)
+    ap.add_argument("--train-steps", type=int, default=3000,
+                    help="Adam steps of the reacting workload's surrogate training")
+    ap.add_argument("--reacting-paths", choices=["both", "fast", "plain"], default="both",
+                    help="with --only reacting: run one path only (profiling passes)")
     ap.add_argument("--element-path", choices=["auto", "mfma", "vector"], default="auto",
                     help="TGV element operators on the matrix cores or the vector units "
                          "(auto: the library default, vector units)")
@@ -805,8 +816,12 @@ def main():
     if args.only == "chem":
         print(json.dumps(chemistry_rate(1 << 20)))
         return
+    paths = {"both": ("f64_plain", "f32_surrogate"), "fast": ("f32_surrogate",),
+             "plain": ("f64_plain",)}[args.reacting_paths]
+    if args.only != "reacting":
+        paths = ("f64_plain", "f32_surrogate")
     r = reacting_rate(n=args.reacting_n, steps=args.steps, warmup=args.warmup,
-                      error_steps=args.error_steps)
+                      error_steps=args.error_steps, train_steps=args.train_steps, paths=paths)
     if args.only == "reacting":
         print(json.dumps(r))
         return

@@ -31,12 +31,14 @@ constexpr double P_STD = 101325.0;
 //   [11, 11 + 17*11)     NASA-7 per species: t_low, t_mid, t_high, low[7], high[7]
 //   then per reaction 6: A, b, theta, third_body, fwd_mode, rev_mode
 //   then nu' (22x11), nu'' (22x11), alpha (22x11)
+//   then per reaction 3: ln A, sum_s (nu''_s - nu'_s), sum_s M_s (nu''_s - nu'_s)
 constexpr int OFF_NASA = NS;
 constexpr int OFF_RX = OFF_NASA + 17 * NS;
 constexpr int OFF_NU1 = OFF_RX + 6 * NR;
 constexpr int OFF_NU2 = OFF_NU1 + NR * NS;
 constexpr int OFF_ALPHA = OFF_NU2 + NR * NS;
-constexpr int TABLE_SIZE = OFF_ALPHA + NR * NS;
+constexpr int OFF_DERIVED = OFF_ALPHA + NR * NS;
+constexpr int TABLE_SIZE = OFF_DERIVED + 3 * NR;
 
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
@@ -102,6 +104,38 @@ __device__ __forceinline__ double recover_temperature(const double* tab, const d
     }
     double dT = -(e - e_target) / dmax(cv, 1.0e-5);
     T = clampd(T + clampd(dT, -0.5 * T, 0.5 * T), 100.0, 50000.0);
+  }
+  return T;
+}
+
+// recover_temperature for the error check: the same damped Newton steps,
+// stopped once a step moves T by at most 1e-13 T (further steps of the fixed
+// count would change T below that) or after max_iters.
+__device__ __forceinline__ double recover_temperature_converged(const double* tab,
+                                                                const double* rho,
+                                                                double e_target, double T,
+                                                                int max_iters) {
+  double tot = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) tot += rho[s];
+  tot = dmax(tot, 1.0e-14);
+  double Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Y[s] = dmax(rho[s], 0.0) / tot;
+  for (int it = 0; it < max_iters; ++it) {
+    double Ts = clampd(T, 50.0, 50000.0);
+    double e = 0.0, cv = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      Thermo th = nasa(tab + OFF_NASA + 17 * s, Ts);
+      e += Y[s] * ((th.h - 1.0) * (R_UNIV * Ts / tab[s]));
+      cv += Y[s] * ((th.cp - 1.0) * (R_UNIV / tab[s]));
+    }
+    double dT = -(e - e_target) / dmax(cv, 1.0e-5);
+    const double Tn = clampd(T + clampd(dT, -0.5 * T, 0.5 * T), 100.0, 50000.0);
+    const bool done = fabs(Tn - T) <= 1.0e-13 * T;
+    T = Tn;
+    if (done) break;
   }
   return T;
 }
@@ -236,15 +270,38 @@ __device__ __forceinline__ void rates(const double* tab, const double* rho, doub
   }
 }
 
+// g = h/RT - s/R of one species at tT > 0 with ln(tT) given: the NASA-7
+// polynomial of nasa(), its logarithm shared when tT lies inside the fit
+// range (the clamped temperature otherwise).
+__device__ __forceinline__ double gibbs(const double* c, double tT, double lT) {
+  const double t = clampd(tT, c[0], c[2]);
+  const double lt = t == tT ? lT : log(t);
+  const double* a = t < c[1] ? c + 3 : c + 10;
+  const double h = a[0] + a[1] * t / 2.0 + a[2] * (t * t) / 3.0 + a[3] * (t * t * t) / 4.0 +
+                   a[4] * (t * t * t * t) / 5.0 + a[5] / t;
+  const double sr = a[0] * lt + a[1] * t + a[2] * (t * t) / 2.0 + a[3] * (t * t * t) / 3.0 +
+                    a[4] * (t * t * t * t) / 4.0 + a[6];
+  return h - sr;
+}
+
 // w = net mass production rates and the diagonal d w_s / d rho_s of their
-// Jacobian at fixed (T, Tv): the arithmetic of rates() without the
-// off-diagonal entries (hybrid/chemistry/mechanism.py:159-220).
-__device__ __forceinline__ void rates_diag(const double* tab, const double* rho, double T,
-                                           double* w, double* dd) {
+// Jacobian at T = Tv (hybrid/chemistry/mechanism.py:159-220), for the
+// integrator's error check.  The arithmetic of rates() rearranged for
+// speed (values equal to rounding): with T = Tv every temperature mode is
+// tT = max(T, 10); k_f fwd and k_r rev are each one exponential of
+// ln A + b ln tT - theta / tT + lf (and of that minus ln K_c plus lr), the
+// clamps of rates() applied to the same arguments; species that take no
+// part in a reaction are skipped (the stoichiometry is uniform across the
+// wave, so these are scalar branches).
+__device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const double* rho,
+                                           double T, double* w, double* dd) {
   const double* mw = tab;
   const double* rx = tab + OFF_RX;
+  const double* der = tab + OFF_DERIVED;
   double C[NS], dC[NS], lc[NS], dlc[NS], g[NS];
   const double tT = dmax(T, 10.0);
+  const double lT = log(tT), iT = 1.0 / tT;
+  const double lP = log(P_STD / (R_UNIV * tT));
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     double x = rho[s] / mw[s];
@@ -253,61 +310,56 @@ __device__ __forceinline__ void rates_diag(const double* tab, const double* rho,
     double cs = dmax(C[s], 1.0e-100);
     lc[s] = log(cs);
     dlc[s] = dmax_dx(C[s], 1.0e-100) / cs;
-    Thermo a = nasa(tab + OFF_NASA + 17 * s, tT);
-    g[s] = a.h - a.s;
+    g[s] = gibbs(tab + OFF_NASA + 17 * s, tT, lT);
   }
-  double om[NS], dom[NS], csum[NS];
+  // the diagonal only scales the error estimate: its accumulators and
+  // factors are FP32 (a non-finite diagonal is caught by the caller)
+  double om[NS];
+  float dom[NS], csum[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) om[s] = dom[s] = csum[s] = 0.0;
+  for (int s = 0; s < NS; ++s) {
+    om[s] = 0.0;
+    dom[s] = csum[s] = 0.0f;
+  }
   for (int r = 0; r < NR; ++r) {
-    const double A = rx[6 * r + 0], b = rx[6 * r + 1], th = rx[6 * r + 2];
+    const double b = rx[6 * r + 1], th = rx[6 * r + 2];
     const bool third = rx[6 * r + 3] != 0.0;
     const double* n1 = tab + OFF_NU1 + r * NS;
     const double* n2 = tab + OFF_NU2 + r * NS;
     const double* al = tab + OFF_ALPHA + r * NS;
-    // T = Tv: every temperature mode is T (sqrt(T Tv) = T for T >= 1)
-    const double tf = dmax(select_temperature((int)rx[6 * r + 4], T, T), 10.0);
-    const double kf = A * pow(tf, b) * exp(-th / tf);
-    const double tr = dmax(select_temperature((int)rx[6 * r + 5], T, T), 10.0);
-    double dG = 0.0, dn = 0.0, lf = 0.0, lr = 0.0, Meff = 0.0;
-    if (tr == tT) {
-#pragma unroll
-      for (int s = 0; s < NS; ++s) dG += (n2[s] - n1[s]) * g[s];
-    } else {
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        Thermo a = nasa(tab + OFF_NASA + 17 * s, tr);
-        dG += (n2[s] - n1[s]) * (a.h - a.s);
-      }
-    }
+    const double lnA = der[3 * r + 0], dn = der[3 * r + 1], mr = der[3 * r + 2];
+    double dG = 0.0, lf = 0.0, lr = 0.0, Meff = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      dn += n2[s] - n1[s];
-      lf += n1[s] * lc[s];
-      lr += n2[s] * lc[s];
-      Meff += al[s] * C[s];
+      const double a1 = n1[s], a2 = n2[s];
+      if (a1 != 0.0 || a2 != 0.0) {
+        dG += (a2 - a1) * g[s];
+        lf += a1 * lc[s];
+        lr += a2 * lc[s];
+      }
+      if (third && al[s] != 0.0) Meff += al[s] * C[s];
     }
-    const double Kc = exp(-clampd(dG, -700.0, 700.0)) * pow(P_STD / (R_UNIV * tr), dn);
-    const double kr = kf / dmax(Kc, 1.0e-300);
-    const double fwd = exp(clampd(lf, -700.0, 700.0));
-    const double rev = exp(clampd(lr, -700.0, 700.0));
+    const double lkf = lnA + b * lT - th * iT;
+    // ln K_c = -clamp(dG) + dn ln(P / R T); k_r = k_f / max(K_c, 1e-300)
+    const double lKc = dmax(-clampd(dG, -700.0, 700.0) + dn * lP, -690.7755278982137);
+    const double fwdk = exp(lkf + clampd(lf, -700.0, 700.0));
+    const double revk = exp(lkf - lKc + clampd(lr, -700.0, 700.0));
     const double cf = fabs(lf) < 700.0 ? 1.0 : (fabs(lf) == 700.0 ? 0.5 : 0.0);
     const double cr = fabs(lr) < 700.0 ? 1.0 : (fabs(lr) == 700.0 ? 0.5 : 0.0);
     const double col = third ? Meff : 1.0;
-    const double q = kf * fwd * col - kr * rev * col;
-    double mr = 0.0;   // mass change of the reaction, sum_k M_k dnu_k
-#pragma unroll
-    for (int s = 0; s < NS; ++s) mr += mw[s] * (n2[s] - n1[s]);
+    const double q = (fwdk - revk) * col;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double d = n2[s] - n1[s];
-      const double dfw = fwd * cf * n1[s] * dlc[s];
-      const double drv = rev * cr * n2[s] * dlc[s];
+      const double a1 = n1[s], a2 = n2[s];
       const double dcol = third ? al[s] : 0.0;
-      const double dq = (kf * (dfw * col + fwd * dcol) - kr * (drv * col + rev * dcol)) * dC[s];
-      om[s] += d * q;
-      dom[s] += d * dq;
-      csum[s] += mr * dq;
+      if (a1 != 0.0 || a2 != 0.0 || dcol != 0.0) {
+        const float dq = (float)((col * (fwdk * cf * a1 - revk * cr * a2) * dlc[s] +
+                                  dcol * (fwdk - revk)) * dC[s]);
+        const double d = a2 - a1;
+        om[s] += d * q;
+        dom[s] += (float)d * dq;
+        csum[s] += (float)mr * dq;
+      }
     }
   }
   double srho = 0.0, S = 0.0;
@@ -324,7 +376,7 @@ __device__ __forceinline__ void rates_diag(const double* tab, const double* rho,
     const double Y = dmax(rho[s], 0.0) / tot;
     w[s] = om[s] * mw[s] - Y * S;
     const double dY = dmax_dx(rho[s], 0.0) / tot - dmax(rho[s], 0.0) / (tot * tot) * dtot;
-    dd[s] = mw[s] * dom[s] - dY * S - Y * csum[s];
+    dd[s] = mw[s] * (double)dom[s] - dY * S - Y * (double)csum[s];
   }
 }
 
@@ -566,19 +618,20 @@ __global__ __launch_bounds__(64) void check_kernel(const double* __restrict__ ta
                                                    double* __restrict__ err_out, int64_t n) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  double r0[NS], rp[NS], Y0[NS], total = 0.0;
+  // rho_0 is read again after the rates rather than held across them
+  double rp[NS], Y0[NS], total = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    r0[s] = dmax(rho_in[c * NS + s], 1.0e-20);
+    Y0[s] = dmax(rho_in[c * NS + s], 1.0e-20);
     rp[s] = rho_p[c * NS + s];
-    total += r0[s];
+    total += Y0[s];
   }
   double T = T_in[c];
   if (!const_T) {
     const double tot = dmax(total, 1.0e-14);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) Y0[s] = r0[s] / tot;
-    T = recover_temperature(tab, rp, mixture_energy(tab, Y0, T), T, 10);
+    for (int s = 0; s < NS; ++s) Y0[s] = Y0[s] / tot;
+    T = recover_temperature_converged(tab, rp, mixture_energy(tab, Y0, T), T, 10);
   }
   const double h = dt_cell ? dt_cell[c] : dt;
   double w[NS], dd[NS];
@@ -587,9 +640,11 @@ __global__ __launch_bounds__(64) void check_kernel(const double* __restrict__ ta
   bool bad = false;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const double R = rp[s] - r0[s] - h * w[s];
-    const double est = fabs(R) / dmax(fabs(1.0 - h * dd[s]), 1.0);
-    const double tol = rtol * fabs(rp[s] - r0[s]) + atol * total;
+    const double r0 = dmax(rho_in[c * NS + s], 1.0e-20);
+    const double R = rp[s] - r0 - h * w[s];
+    const double a = fabs(1.0 - h * dd[s]);
+    const double est = fabs(R) / (a - a == 0.0 ? dmax(a, 1.0) : 1.0);
+    const double tol = rtol * fabs(rp[s] - r0) + atol * total;
     bad = bad || !(est <= tol);
     worst = dmax(worst, est / dmax(total, 1.0e-300));
   }

@@ -158,7 +158,13 @@ def test_device_table_transcription_matches_oracle():
     for arr in (MECH.nu1, MECH.nu2, MECH.alpha):
         assert np.array_equal(tab[o:o + nr * ns].reshape(nr, ns), arr)
         o += nr * ns
-    assert o == len(tab)
+    # derived per reaction: ln A, sum(nu'' - nu'), sum M (nu'' - nu')
+    der = tab[o:o + 3 * nr].reshape(nr, 3)
+    d = MECH.nu2 - MECH.nu1
+    assert np.allclose(np.exp(der[:, 0]), MECH.A, rtol=1e-14, atol=0)
+    assert np.array_equal(der[:, 1], d.sum(1))
+    assert np.allclose(der[:, 2], d @ MOLAR_MASS, rtol=0, atol=1e-15)
+    assert o + 3 * nr == len(tab)
 
 
 def test_be_check_pins():
