@@ -239,8 +239,11 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   __shared__ real sb[N * 12];
   __shared__ real sL[N * 12];
   // per point: the pulled-back flow fluxes [3][5]; with species also w (J_r .
-  // u) [3], w (J_r . u_a) [3] and the species interpolants [V - 5]
-  constexpr int SW = V > 5 ? 15 + 6 + (V - 5) : 3 * V;
+  // u) [3], w (J_r . u_a) [3] and the species interpolants [V - 5].  The
+  // row stride is odd, so that the lanes of a wave (one point each) write
+  // and read distinct banks (21 + 11 = 32 words would put all on one).
+  constexpr int SW0 = V > 5 ? 15 + 6 + (V - 5) : 3 * V;
+  constexpr int SW = SW0 | 1;
   __shared__ real sF[Q * SW];
   __shared__ double red[2][NT / 64];
   __shared__ unsigned int redn[NT / 64];
