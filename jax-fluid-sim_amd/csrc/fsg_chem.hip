@@ -108,9 +108,12 @@ __device__ __forceinline__ double recover_temperature(const double* tab, const d
   return T;
 }
 
-// recover_temperature for the error check: the same damped Newton steps,
-// stopped once a step moves T by at most 1e-13 T (further steps of the fixed
-// count would change T below that) or after max_iters.
+// recover_temperature stopped once a step moves T by at most 1e-13 T
+// (further steps of the fixed count would change T below that) or after
+// max_iters: the error check, and the coupler's initial temperature
+// (hybrid/chemistry/coupling.py:135-154, 8 steps there) in both the fused
+// react_kernel and the split prepare_kernel, which therefore agree bit for
+// bit.
 __device__ __forceinline__ double recover_temperature_converged(const double* tab,
                                                                 const double* rho,
                                                                 double e_target, double T,
@@ -518,13 +521,13 @@ __global__ __launch_bounds__(64) void react_kernel(const double* __restrict__ ta
 #pragma unroll
     for (int s = 0; s < NS; ++s) rp[s] = rs[s] * p.rho_ref;
     double Tg = dmax((p.gamma - 1.0) * eth_s * p.T_ref, 200.0);
-    double T0 = recover_temperature(tab, rp, eth, Tg, 8);
+    double T0 = recover_temperature_converged(tab, rp, eth, Tg, 8);
     integrate(tab, rp, T0, h * p.t_ref, p.newton_iters, 0);
 #pragma unroll
     for (int s = 0; s < NS; ++s) rs[s] = rp[s] / p.rho_ref;
   } else {
     double eth = dmax(u[4] / rt - ek, 1.0e3);
-    double T0 = recover_temperature(tab, rs, eth, 1000.0, 8);
+    double T0 = recover_temperature_converged(tab, rs, eth, 1000.0, 8);
     integrate(tab, rs, T0, h, p.newton_iters, 0);
   }
   double s2 = 0.0;
@@ -568,10 +571,10 @@ __global__ __launch_bounds__(64) void prepare_kernel(const double* __restrict__ 
 #pragma unroll
     for (int s = 0; s < NS; ++s) rs[s] *= p.rho_ref;
     const double Tg = dmax((p.gamma - 1.0) * eth_s * p.T_ref, 200.0);
-    T0 = recover_temperature(tab, rs, eth_s * p.e_ref, Tg, 8);
+    T0 = recover_temperature_converged(tab, rs, eth_s * p.e_ref, Tg, 8);
   } else {
     const double eth = dmax((double)u[4] / rt - ek, 1.0e3);
-    T0 = recover_temperature(tab, rs, eth, 1000.0, 8);
+    T0 = recover_temperature_converged(tab, rs, eth, 1000.0, 8);
   }
 #pragma unroll
   for (int s = 0; s < NS; ++s) rho_out[c * NS + s] = rs[s];
