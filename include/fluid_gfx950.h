@@ -222,6 +222,9 @@ const char* fsg_last_error(void);
  * the element kernel uses (no counterpart in the reference: a device check
  * of the operand / accumulator lane layout). */
 int fsg_mfma_selftest(int32_t dtype, const void* A, const void* B, void* C, void* stream);
+/* Version of the structure layouts and entry points of this header
+ * (FSG_ABI_VERSION); a caller built against another version must not bind. */
+#define FSG_ABI_VERSION 2
 int32_t fsg_abi_version(void);
 
 /* ---- XLA custom-call target -------------------------------------------------

@@ -366,6 +366,6 @@ void fsg_destroy(fsg_context* ctx) {
 
 const char* fsg_last_error(void) { return g_error.c_str(); }
 
-int32_t fsg_abi_version(void) { return 1; }
+int32_t fsg_abi_version(void) { return FSG_ABI_VERSION; }
 
 }  // extern "C"

@@ -112,8 +112,13 @@ class Boundary(ctypes.Structure):
 _lib = None
 
 
+# FSG_ABI_VERSION of include/fluid_gfx950.h that these structures mirror
+ABI_VERSION = 2
+
+
 def lib():
-    """The loaded library; raises when it has not been built."""
+    """The loaded library; raises when it has not been built or was built
+    against another ABI version."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
@@ -144,6 +149,9 @@ def lib():
         L.fsg_last_error.argtypes = []
         L.fsg_last_error.restype = ctypes.c_char_p
         L.fsg_abi_version.restype = ctypes.c_int32
+        if L.fsg_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.fsg_abi_version()}, these bindings "
+                               f"{ABI_VERSION}: rebuild it with jax-fluid-sim_amd/build.py")
         L.fsg_chem_table_size.restype = ctypes.c_int32
         L.fsg_chem_integrate.argtypes = [vp, vp, vp, vp, f64, i32, i32, vp, vp, i64, vp]
         L.fsg_chem_integrate.restype = ctypes.c_int

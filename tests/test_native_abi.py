@@ -25,7 +25,9 @@ def test_library_exports_every_declared_symbol():
     assert "fsg_create" in declared and "fsg_rk_stage" in declared
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.fsg_abi_version() >= 1
+    header = (ROOT / "include" / "fluid_gfx950.h").read_text()
+    want = int(re.search(r"#define FSG_ABI_VERSION (\d+)", header).group(1))
+    assert lib.fsg_abi_version() == want == nat.ABI_VERSION
 
 
 def test_context_over_cpu_tables_and_argument_checks():
