@@ -18,6 +18,7 @@
 //     combination and admissibility reductions of
 //     hybrid/time_integration.py:264-312, 404-439.
 #pragma once
+#include "fsg_lanes.h"
 #include "fsg_physics.h"
 
 namespace fsg {
@@ -448,11 +449,8 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
         }
       }
 #pragma unroll
-    for (int o = 1; o < DS; o <<= 1) {
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], o, 64);
-      if constexpr (V > VF) acc_a += __shfl_xor(acc_a, o, 64);
-    }
+    for (int v = 0; v < V; ++v) acc[v] = group_sum<DS>(acc[v]);
+    if constexpr (V > VF) acc_a = group_sum<DS>(acc_a);
     if (i < N && h == 0) {
 #pragma unroll
       for (int v = 0; v < VF; ++v) sR[i * V + v] += acc[v];
@@ -972,10 +970,8 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
       }
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        acc_o[m] += __shfl_xor(acc_o[m], 1, 64);
-        acc_o[m] += __shfl_xor(acc_o[m], 2, 64);
-        acc_n[m] += __shfl_xor(acc_n[m], 1, 64);
-        acc_n[m] += __shfl_xor(acc_n[m], 2, 64);
+        acc_o[m] = group_sum<4>(acc_o[m]);
+        acc_n[m] = group_sum<4>(acc_n[m]);
       }
       static_assert(FO <= 16 && FN <= 16, "four lanes per support node in one wave");
       if (part == 0 && kk < FO)
@@ -1058,10 +1054,8 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
           }
 #pragma unroll
           for (int k = 0; k < 12; ++k) {
-            G[k] += __shfl_xor(G[k], 1, 64);
-            G[k] += __shfl_xor(G[k], 2, 64);
-            L[k] += __shfl_xor(L[k], 1, 64);
-            L[k] += __shfl_xor(L[k], 2, 64);
+            G[k] = group_sum<4>(G[k]);
+            L[k] = group_sum<4>(L[k]);
           }
           if (q < QF && part == 0) {
             real raw[12];
@@ -1252,10 +1246,8 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
     }
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      ao[v] += __shfl_xor(ao[v], 1, 64);
-      ao[v] += __shfl_xor(ao[v], 2, 64);
-      an[v] += __shfl_xor(an[v], 1, 64);
-      an[v] += __shfl_xor(an[v], 2, 64);
+      ao[v] = group_sum<4>(ao[v]);
+      an[v] = group_sum<4>(an[v]);
     }
     real* oo = a.out_o + f * (FO * W);
     if (part == 0 && kk < FO)
