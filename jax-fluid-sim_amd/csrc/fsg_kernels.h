@@ -854,27 +854,25 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
         }
       };
       if (euler) {
-        real F[5], Bo[5];
+        real F[5], Bo[5], Bn[5];
         FluxScalars<real> fs;
         numflux_split<V, real>(UL, UR, n, gamma, ph.rusanov, sl, sr, F, fs);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Bo[v] = sB[0][v];
-        const real una = sB[0][5];
+        for (int v = 0; v < 5; ++v) {
+          Bo[v] = sB[0][v];
+          Bn[v] = NBR ? sB[1][v] : real(0);
+        }
+        const real una = sB[0][5], unb = NBR ? sB[1][5] : real(0);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) sSo[q * V + v] = -w * (F[v] - Bo[v]);
-        for (int sp = 5; sp < V; ++sp)
-          sSo[q * V + sp] = -w * (species_flux(fs, sl(sp), sr(sp)) - sUo[sp] * una);
-        if constexpr (NBR) {
-          real Bn[5];
-#pragma unroll
-          for (int v = 0; v < 5; ++v) Bn[v] = sB[1][v];
-          const real unb = sB[1][5];
-#pragma unroll
-          for (int v = 0; v < 5; ++v) sSn[q * V + v] = w * (F[v] - Bn[v]);
-          for (int sp = 5; sp < V; ++sp)
-            sSn[q * V + sp] = w * (species_flux(fs, sl(sp), sr(sp)) - sUn[sp] * unb);
-        } else {
-          for (int v = 0; v < V; ++v) sSn[q * V + v] = real(0);
+        for (int v = 0; v < 5; ++v) {
+          sSo[q * V + v] = -w * (F[v] - Bo[v]);
+          sSn[q * V + v] = NBR ? w * (F[v] - Bn[v]) : real(0);
+        }
+        // each species column's flux once, for both sides
+        for (int sp = 5; sp < V; ++sp) {
+          const real fsp = species_flux(fs, sl(sp), sr(sp));
+          sSo[q * V + sp] = -w * (fsp - sUo[sp] * una);
+          sSn[q * V + sp] = NBR ? w * (fsp - sUn[sp] * unb) : real(0);
         }
       } else {
         for (int v = 0; v < V; ++v) {

@@ -240,28 +240,30 @@ __device__ __forceinline__ void hllc(const real* UL, const real* UR, const real*
 }
 
 // The numerical flux of hllc / rusanov above with the species columns kept
-// out of registers (states with many passive species).  The flow part
-// (columns 0-4) is computed here; every species column follows from the
-// scalars in FluxScalars and that column's two states alone, by the same
-// expressions as the all-column functions, so results are bit-identical.
-// sl(s) / sr(s) return the left / right state of species column s (s >= 5).
+// out of the state registers (states with many passive species).  The flow
+// part (columns 0-4) goes to F; every species column follows from the
+// scalars in FluxScalars and that column's two states alone
+// (species_flux), by the same expressions as the all-column functions, so
+// results are bit-identical.  sl(s) / sr(s) return the left / right state of
+// species column s (s >= 5).
 template <typename real>
 struct FluxScalars {
   int mode;     // 0: F_L, 1: left star, 2: right star, 3: F_R, 4: Rusanov
   real unl, unr, S, comp, smax;
 };
 
+// The species column of each mode (F_L, left star, right star, F_R,
+// Rusanov) as selects between the mode expressions, without branches: the
+// lanes of a wave take different modes.
 template <typename real>
 __device__ __forceinline__ real species_flux(const FluxScalars<real>& f, real uL, real uR) {
 #pragma clang fp contract(off)
   const real FL = uL * f.unl, FR = uR * f.unr;
-  switch (f.mode) {
-    case 0: return FL;
-    case 1: return FL + f.S * (uL * f.comp - uL);
-    case 2: return FR + f.S * (uR * f.comp - uR);
-    case 3: return FR;
-    default: return real(0.5) * (FL + FR) - real(0.5) * f.smax * (uR - uL);
-  }
+  const bool right = f.mode == 2;
+  const real u = right ? uR : uL, F = right ? FR : FL;
+  const real star = F + f.S * (u * f.comp - u);
+  const real rus = real(0.5) * (FL + FR) - real(0.5) * f.smax * (uR - uL);
+  return f.mode == 0 ? FL : (f.mode == 3 ? FR : (f.mode == 4 ? rus : star));
 }
 
 template <int V, typename real>
@@ -297,7 +299,8 @@ __device__ __forceinline__ void numflux_split(const real* UL, const real* UR, co
   bool same = true;
 #pragma unroll
   for (int v = 0; v < 5; ++v) same = same && (UL[v] == UR[v]);
-  for (int s = 5; s < V; ++s) same = same && (sl(s) == sr(s));
+  // non-short-circuit accumulations (no branch per column)
+  for (int s = 5; s < V; ++s) same = same & (sl(s) == sr(s));
   f.mode = 0;
   if (same) {
 #pragma unroll
@@ -335,7 +338,7 @@ __device__ __forceinline__ void numflux_split(const real* UL, const real* UR, co
   bool ok = (wl.rho > real(0)) && (wr.rho > real(0)) && (wl.p > real(0)) && (wr.p > real(0));
 #pragma unroll
   for (int v = 0; v < 5; ++v) ok = ok && finite(H[v]);
-  for (int s = 5; s < V; ++s) ok = ok && finite(species_flux(f, sl(s), sr(s)));
+  for (int s = 5; s < V; ++s) ok = ok & finite(species_flux(f, sl(s), sr(s)));
   if (ok) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) F[v] = H[v];
