@@ -32,13 +32,17 @@ constexpr double P_STD = 101325.0;
 //   then per reaction 6: A, b, theta, third_body, fwd_mode, rev_mode
 //   then nu' (22x11), nu'' (22x11), alpha (22x11)
 //   then per reaction 3: ln A, sum_s (nu''_s - nu'_s), sum_s M_s (nu''_s - nu'_s)
+//   then per species 18: 1/M, R/M, 1/t_low, 1/t_high, and per fit range
+//   (low, high) a1/2, a2/3, a3/4, a4/5 (h/RT) and a2/2, a3/3, a4/4 (s/R)
 constexpr int OFF_NASA = NS;
 constexpr int OFF_RX = OFF_NASA + 17 * NS;
 constexpr int OFF_NU1 = OFF_RX + 6 * NR;
 constexpr int OFF_NU2 = OFF_NU1 + NR * NS;
 constexpr int OFF_ALPHA = OFF_NU2 + NR * NS;
 constexpr int OFF_DERIVED = OFF_ALPHA + NR * NS;
-constexpr int TABLE_SIZE = OFF_DERIVED + 3 * NR;
+constexpr int OFF_SPEC = OFF_DERIVED + 3 * NR;
+constexpr int SPEC_W = 18;
+constexpr int TABLE_SIZE = OFF_SPEC + SPEC_W * NS;
 
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
@@ -48,72 +52,96 @@ __device__ __forceinline__ double dmax_dx(double x, double y) {
   return x > y ? 1.0 : (x == y ? 0.5 : 0.0);
 }
 
-// NASA-7 with T clamped to [t_low, t_high] inside the polynomial
-// (hybrid/chemistry/nasa.py:47-119).
+// NASA-7 (hybrid/chemistry/nasa.py:47-119) with T clamped to [t_low, t_high]
+// inside the polynomial, in Horner form with the divided coefficients of the
+// species' derived block (a1/2, a2/3, ... written by the host, so no
+// division by 3 or 5 per evaluation) and 1/t given: the reference's power
+// sums to rounding.  The coefficients are read with scalar (wave-uniform)
+// loads: each fit range is evaluated when some lane of the wave lies in it
+// (a uniform branch; the lanes of a wave almost always share a range) and
+// every lane selects its own.
 struct Thermo {
-  double cp, h, s;  // cp/R, h/(RT), s/R
+  double cp, h;  // cp/R, h/(RT)
 };
 
-__device__ __forceinline__ Thermo nasa(const double* c, double T) {
-  double t = clampd(T, c[0], c[2]);
-  const double* a = t < c[1] ? c + 3 : c + 10;
-  Thermo r;
-  r.cp = a[0] + a[1] * t + a[2] * (t * t) + a[3] * (t * t * t) + a[4] * (t * t * t * t);
-  r.h = a[0] + a[1] * t / 2.0 + a[2] * (t * t) / 3.0 + a[3] * (t * t * t) / 4.0 +
-        a[4] * (t * t * t * t) / 5.0 + a[5] / t;
-  r.s = a[0] * log(t) + a[1] * t + a[2] * (t * t) / 2.0 + a[3] * (t * t * t) / 3.0 +
-        a[4] * (t * t * t * t) / 4.0 + a[6];
-  return r;
+__device__ __forceinline__ double clamped_inverse(const double* c, const double* sp, double t,
+                                                  double T, double iT) {
+  return t == T ? iT : (t == c[0] ? sp[2] : sp[3]);
 }
 
-__device__ __forceinline__ double select_temperature(int mode, double T, double Tv) {
-  double Ta = sqrt(dmax(T * Tv, 1.0));
-  return mode == 1 ? Ta : (mode == 2 ? Tv : T);
+__device__ __forceinline__ void nasa_range(const double* a, const double* d, double t, double it,
+                                           double& cp, double& h) {
+  cp = a[0] + t * (a[1] + t * (a[2] + t * (a[3] + t * a[4])));
+  h = a[0] + t * (d[0] + t * (d[1] + t * (d[2] + t * d[3]))) + a[5] * it;
+}
+
+__device__ __forceinline__ Thermo nasa(const double* c, const double* sp, double T, double iT) {
+  const double t = clampd(T, c[0], c[2]);
+  const double it = clamped_inverse(c, sp, t, T, iT);
+  const bool lo = t < c[1];
+  double cpl = 0.0, hl = 0.0, cph = 0.0, hh = 0.0;
+  if (__any(lo)) nasa_range(c + 3, sp + 4, t, it, cpl, hl);
+  if (__any(!lo)) nasa_range(c + 10, sp + 11, t, it, cph, hh);
+  return Thermo{lo ? cpl : cph, lo ? hl : hh};
+}
+
+// g = h/RT - s/R of one species at tT > 0 with ln(tT) and 1/tT given (the
+// logarithm of the clamped temperature otherwise).
+__device__ __forceinline__ double gibbs_range(const double* a, const double* d, double t, double lt,
+                                              double it) {
+  const double h = a[0] + t * (d[0] + t * (d[1] + t * (d[2] + t * d[3]))) + a[5] * it;
+  const double sr = a[0] * lt + t * (a[1] + t * (d[4] + t * (d[5] + t * d[6]))) + a[6];
+  return h - sr;
+}
+
+__device__ __forceinline__ double gibbs(const double* c, const double* sp, double tT, double lT,
+                                        double iT) {
+  const double t = clampd(tT, c[0], c[2]);
+  const double lt = t == tT ? lT : log(t);
+  const double it = clamped_inverse(c, sp, t, tT, iT);
+  const bool lo = t < c[1];
+  double gl = 0.0, gh = 0.0;
+  if (__any(lo)) gl = gibbs_range(c + 3, sp + 4, t, lt, it);
+  if (__any(!lo)) gh = gibbs_range(c + 10, sp + 11, t, lt, it);
+  return lo ? gl : gh;
+}
+
+// The table pointer offset by a zero the compiler cannot see through:
+// addresses formed from it inside a loop stay inside the loop (scalar loads
+// with immediate offsets from one base) instead of being hoisted as a
+// hundred 64-bit sums that spill.  The offset is wave-uniform ("s").
+__device__ __forceinline__ const double* table_launder(const double* p) {
+  int zero = 0;
+  __asm__ volatile("" : "+s"(zero));
+  return p + zero;
+}
+
+__device__ __forceinline__ const double* species_block(const double* tab, int s) {
+  return tab + OFF_SPEC + SPEC_W * s;
 }
 
 // e_mix(T) with h = h/RT * R T / M_s (T unclamped outside), as
 // hybrid/chemistry/mechanism.py:222-260.
 __device__ __forceinline__ double mixture_energy(const double* tab, const double* Y, double T) {
+  const double iT = 1.0 / T;
   double e = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    Thermo th = nasa(tab + OFF_NASA + 17 * s, T);
-    e += Y[s] * ((th.h - 1.0) * (R_UNIV * T / tab[s]));
+    const double* sp = species_block(tab, s);
+    Thermo th = nasa(tab + OFF_NASA + 17 * s, sp, T, iT);
+    e += Y[s] * ((th.h - 1.0) * (sp[1] * T));
   }
   return e;
 }
 
-// hybrid/chemistry/mechanism.py:262-287
-__device__ __forceinline__ double recover_temperature(const double* tab, const double* rho, double e_target,
-                                      double T, int iters) {
-  double tot = 0.0;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) tot += rho[s];
-  tot = dmax(tot, 1.0e-14);
-  double Y[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) Y[s] = dmax(rho[s], 0.0) / tot;
-  for (int it = 0; it < iters; ++it) {
-    double Ts = clampd(T, 50.0, 50000.0);
-    double e = 0.0, cv = 0.0;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      Thermo th = nasa(tab + OFF_NASA + 17 * s, Ts);
-      e += Y[s] * ((th.h - 1.0) * (R_UNIV * Ts / tab[s]));
-      cv += Y[s] * ((th.cp - 1.0) * (R_UNIV / tab[s]));
-    }
-    double dT = -(e - e_target) / dmax(cv, 1.0e-5);
-    T = clampd(T + clampd(dT, -0.5 * T, 0.5 * T), 100.0, 50000.0);
-  }
-  return T;
-}
-
-// recover_temperature stopped once a step moves T by at most 1e-13 T
-// (further steps of the fixed count would change T below that) or after
-// max_iters: the error check, and the coupler's initial temperature
-// (hybrid/chemistry/coupling.py:135-154, 8 steps there) in both the fused
-// react_kernel and the split prepare_kernel, which therefore agree bit for
-// bit.
+// The damped Newton T(e) of hybrid/chemistry/mechanism.py:262-287, stopped
+// once a step moves T by at most 1e-13 T (further steps of the fixed count
+// would change T below that) or after max_iters.  Used for the coupler's
+// initial temperature (hybrid/chemistry/coupling.py:135-154, 8 steps there),
+// the integrator's temperature after each Newton iteration
+// (hybrid/chemistry/integrator.py:88-95, 10 steps there) and the error
+// check; the fused react_kernel and the split prepare_kernel therefore agree
+// bit for bit.
 __device__ __forceinline__ double recover_temperature_converged(const double* tab,
                                                                 const double* rho,
                                                                 double e_target, double T,
@@ -122,17 +150,22 @@ __device__ __forceinline__ double recover_temperature_converged(const double* ta
 #pragma unroll
   for (int s = 0; s < NS; ++s) tot += rho[s];
   tot = dmax(tot, 1.0e-14);
+  const double itot = 1.0 / tot;
   double Y[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) Y[s] = dmax(rho[s], 0.0) / tot;
+  for (int s = 0; s < NS; ++s) Y[s] = dmax(rho[s], 0.0) * itot;
+#pragma unroll 1
   for (int it = 0; it < max_iters; ++it) {
-    double Ts = clampd(T, 50.0, 50000.0);
+    const double* tb = table_launder(tab);
+    const double Ts = clampd(T, 50.0, 50000.0);
+    const double iTs = 1.0 / Ts;
     double e = 0.0, cv = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      Thermo th = nasa(tab + OFF_NASA + 17 * s, Ts);
-      e += Y[s] * ((th.h - 1.0) * (R_UNIV * Ts / tab[s]));
-      cv += Y[s] * ((th.cp - 1.0) * (R_UNIV / tab[s]));
+      const double* sp = species_block(tb, s);
+      Thermo th = nasa(tb + OFF_NASA + 17 * s, sp, Ts, iTs);
+      e += Y[s] * ((th.h - 1.0) * (sp[1] * Ts));
+      cv += Y[s] * ((th.cp - 1.0) * sp[1]);
     }
     double dT = -(e - e_target) / dmax(cv, 1.0e-5);
     const double Tn = clampd(T + clampd(dT, -0.5 * T, 0.5 * T), 100.0, 50000.0);
@@ -143,148 +176,122 @@ __device__ __forceinline__ double recover_temperature_converged(const double* ta
   return T;
 }
 
-// Net mass production rates w and d w / d rho at fixed (T, Tv)
-// (hybrid/chemistry/mechanism.py:159-220).
-__device__ __forceinline__ void rates(const double* tab, const double* rho, double T, double Tv, double* w,
-                      double (*dw)[NS]) {
+// Net mass production rates w and their Jacobian d w / d rho at T = Tv,
+// the integrator's case (hybrid/chemistry/mechanism.py:159-220, the
+// derivative jax.jacobian takes, with its half gradient at max(x, y) ties).
+// With T = Tv every temperature mode is tT = max(T, 10).  The arithmetic is
+// rearranged (values equal to rounding): k_f fwd and k_r rev are each one
+// exponential of ln A + b ln tT - theta / tT + lf (and of that minus ln K_c
+// plus lr), with the clamps of the reference on the same arguments
+// (ln K_c >= ln 1e-300 for k_r = k_f / max(K_c, 1e-300)); the Gibbs
+// energies share log tT; species that take no part in a reaction are
+// skipped (the stoichiometry is uniform across the wave: scalar branches);
+// the Jacobian's column sums come from the reactions' mass balances sum_s
+// M_s (nu''_s - nu'_s); the mass-sum correction w - Y sum(w) is
+// differentiated with one reciprocal of the total.
+__device__ __forceinline__ void rates(const double* __restrict__ tab, const double* rho, double T,
+                                      double* w, double (*dw)[NS]) {
   const double* mw = tab;
-  const double* rx = tab + OFF_RX;
-  double C[NS], dC[NS], lc[NS], dlc[NS];
+  double C[NS], dC[NS], lc[NS], dlc[NS], g[NS];
+  const double tT = dmax(T, 10.0);
+  const double lT = log(tT), iT = 1.0 / tT;
+  const double lP = log(P_STD / (R_UNIV * tT));
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    double x = rho[s] / mw[s];
+    const double* sp = species_block(tab, s);
+    const double x = rho[s] * sp[0];
     C[s] = dmax(x, 0.0);
-    dC[s] = dmax_dx(x, 0.0) / mw[s];
-    double cs = dmax(C[s], 1.0e-100);
+    dC[s] = dmax_dx(x, 0.0) * sp[0];
+    const double cs = dmax(C[s], 1.0e-100);
     lc[s] = log(cs);
     dlc[s] = dmax_dx(C[s], 1.0e-100) / cs;
+    g[s] = gibbs(tab + OFF_NASA + 17 * s, sp, tT, lT, iT);
   }
-  // Gibbs energies at T and at Tv (the reverse temperatures used here).
-  double gT[NS], gV[NS];
-  const double tT = dmax(T, 10.0), tV = dmax(Tv, 10.0);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    Thermo a = nasa(tab + OFF_NASA + 17 * s, tT);
-    gT[s] = a.h - a.s;
-    if (Tv == T) {
-      gV[s] = gT[s];
-    } else {
-      Thermo b = nasa(tab + OFF_NASA + 17 * s, tV);
-      gV[s] = b.h - b.s;
-    }
-  }
-  // dw accumulates M_s d omega_s / d rho_j directly (no separate omega
-  // Jacobian held in registers).
-  double om[NS];
+  double om[NS], csum[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     om[s] = 0.0;
+    csum[s] = 0.0;
 #pragma unroll
     for (int j = 0; j < NS; ++j) dw[s][j] = 0.0;
   }
+#pragma unroll 1
   for (int r = 0; r < NR; ++r) {
-    const double A = rx[6 * r + 0], b = rx[6 * r + 1], th = rx[6 * r + 2];
-    const bool third = rx[6 * r + 3] != 0.0;
-    const int fm = (int)rx[6 * r + 4], rm = (int)rx[6 * r + 5];
-    const double* n1 = tab + OFF_NU1 + r * NS;
-    const double* n2 = tab + OFF_NU2 + r * NS;
-    const double* al = tab + OFF_ALPHA + r * NS;
-    double tf = dmax(select_temperature(fm, T, Tv), 10.0);
-    double kf = A * pow(tf, b) * exp(-th / tf);
-    double tr = dmax(select_temperature(rm, T, Tv), 10.0);
-    double dG = 0.0, dn = 0.0;
-    // reverse modes are T or Tv in this mechanism; Ta falls back to a fresh
-    // Gibbs evaluation at that temperature.
-    if (rm == 1) {
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        Thermo a = nasa(tab + OFF_NASA + 17 * s, tr);
-        dG += (n2[s] - n1[s]) * (a.h - a.s);
-      }
-    } else {
-      const double* g = rm == 2 ? gV : gT;
-#pragma unroll
-      for (int s = 0; s < NS; ++s) dG += (n2[s] - n1[s]) * g[s];
-    }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) dn += n2[s] - n1[s];
-    double Kc = exp(-clampd(dG, -700.0, 700.0)) * pow(P_STD / (R_UNIV * tr), dn);
-    double kr = kf / dmax(Kc, 1.0e-300);
-    double lf = 0.0, lr = 0.0, Meff = 0.0;
+    const double* tb = table_launder(tab);
+    const double* rx = tb + OFF_RX + 6 * r;
+    const double b = rx[1], th = rx[2];
+    const bool third = rx[3] != 0.0;
+    const double* n1 = tb + OFF_NU1 + r * NS;
+    const double* n2 = tb + OFF_NU2 + r * NS;
+    const double* al = tb + OFF_ALPHA + r * NS;
+    const double* dr = tb + OFF_DERIVED + 3 * r;
+    const double lnA = dr[0], dn = dr[1], mr = dr[2];
+    double dG = 0.0, lf = 0.0, lr = 0.0, Meff = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      lf += n1[s] * lc[s];
-      lr += n2[s] * lc[s];
-      Meff += al[s] * C[s];
+      const double a1 = n1[s], a2 = n2[s];
+      if (a1 != 0.0 || a2 != 0.0) {
+        dG += (a2 - a1) * g[s];
+        lf += a1 * lc[s];
+        lr += a2 * lc[s];
+      }
+      if (third && al[s] != 0.0) Meff += al[s] * C[s];
     }
-    double fwd = exp(clampd(lf, -700.0, 700.0));
-    double rev = exp(clampd(lr, -700.0, 700.0));
-    double cf = fabs(lf) < 700.0 ? 1.0 : (fabs(lf) == 700.0 ? 0.5 : 0.0);
-    double cr = fabs(lr) < 700.0 ? 1.0 : (fabs(lr) == 700.0 ? 0.5 : 0.0);
-    double col = third ? Meff : 1.0;
-    double q = kf * fwd * col - kr * rev * col;
+    const double lkf = lnA + b * lT - th * iT;
+    const double lKc = dmax(-clampd(dG, -700.0, 700.0) + dn * lP, -690.7755278982137);
+    const double fwdk = exp(lkf + clampd(lf, -700.0, 700.0));
+    const double revk = exp(lkf - lKc + clampd(lr, -700.0, 700.0));
+    const double cf = fabs(lf) < 700.0 ? 1.0 : (fabs(lf) == 700.0 ? 0.5 : 0.0);
+    const double cr = fabs(lr) < 700.0 ? 1.0 : (fabs(lr) == 700.0 ? 0.5 : 0.0);
+    const double col = third ? Meff : 1.0;
+    const double net = fwdk - revk;
+    const double q = net * col;
+    const double fc = col * fwdk * cf, rc = col * revk * cr;
     double dq[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      double dfw = fwd * cf * n1[s] * dlc[s];
-      double drv = rev * cr * n2[s] * dlc[s];
-      double dcol = third ? al[s] : 0.0;
-      dq[s] = (kf * (dfw * col + fwd * dcol) - kr * (drv * col + rev * dcol)) * dC[s];
+    for (int j = 0; j < NS; ++j) {
+      const double a1 = n1[j], a2 = n2[j];
+      const double dcol = third ? al[j] : 0.0;
+      dq[j] = 0.0;
+      if (a1 != 0.0 || a2 != 0.0 || dcol != 0.0)
+        dq[j] = ((fc * a1 - rc * a2) * dlc[j] + dcol * net) * dC[j];
+      csum[j] += mr * dq[j];
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      double d = n2[s] - n1[s];
+      const double d = n2[s] - n1[s];
       if (d != 0.0) {
         om[s] += d * q;
-        const double md = mw[s] * d;
+        const double md = tb[s] * d;
 #pragma unroll
         for (int j = 0; j < NS; ++j) dw[s][j] += md * dq[j];
       }
     }
   }
-  // w = M omega, then the exact mass-sum correction w - Y sum(w).
+  // w = M omega, then the exact mass-sum correction w - Y sum(w)
   double srho = 0.0, S = 0.0;
-  double wr[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     srho += rho[s];
-    wr[s] = om[s] * mw[s];
-    S += wr[s];
+    S += om[s] * mw[s];
   }
-  double tot = dmax(srho, 1.0e-14);
-  double dtot = dmax_dx(srho, 1.0e-14);
-  double colsum[NS];
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    colsum[j] = 0.0;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) colsum[j] += dw[s][j];
-  }
+  const double tot = dmax(srho, 1.0e-14), dtot = dmax_dx(srho, 1.0e-14);
+  const double itot = 1.0 / tot;
+  const double k2 = dtot * (itot * itot);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    double Y = dmax(rho[s], 0.0) / tot;
-    w[s] = wr[s] - Y * S;
-    double dpos = dmax_dx(rho[s], 0.0);
+    const double rp = dmax(rho[s], 0.0);
+    const double Y = rp * itot;
+    w[s] = om[s] * mw[s] - Y * S;
+    // d Y_s / d rho_j = delta_sj [rho_s > 0] / tot - rho_s dtot / tot^2
+    const double off = rp * k2;
+    const double diag = dmax_dx(rho[s], 0.0) * itot;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      double dY = (s == j ? dpos / tot : 0.0) - dmax(rho[s], 0.0) / (tot * tot) * dtot;
-      dw[s][j] = dw[s][j] - dY * S - Y * colsum[j];
+      const double dY = (s == j ? diag : 0.0) - off;
+      dw[s][j] = dw[s][j] - dY * S - Y * csum[j];
     }
   }
-}
-
-// g = h/RT - s/R of one species at tT > 0 with ln(tT) given: the NASA-7
-// polynomial of nasa(), its logarithm shared when tT lies inside the fit
-// range (the clamped temperature otherwise).
-__device__ __forceinline__ double gibbs(const double* c, double tT, double lT) {
-  const double t = clampd(tT, c[0], c[2]);
-  const double lt = t == tT ? lT : log(t);
-  const double* a = t < c[1] ? c + 3 : c + 10;
-  const double h = a[0] + a[1] * t / 2.0 + a[2] * (t * t) / 3.0 + a[3] * (t * t * t) / 4.0 +
-                   a[4] * (t * t * t * t) / 5.0 + a[5] / t;
-  const double sr = a[0] * lt + a[1] * t + a[2] * (t * t) / 2.0 + a[3] * (t * t * t) / 3.0 +
-                    a[4] * (t * t * t * t) / 4.0 + a[6];
-  return h - sr;
 }
 
 // w = net mass production rates and the diagonal d w_s / d rho_s of their
@@ -307,13 +314,14 @@ __device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const
   const double lP = log(P_STD / (R_UNIV * tT));
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    double x = rho[s] / mw[s];
+    const double* sp = species_block(tab, s);
+    const double x = rho[s] * sp[0];
     C[s] = dmax(x, 0.0);
-    dC[s] = dmax_dx(x, 0.0) / mw[s];
+    dC[s] = dmax_dx(x, 0.0) * sp[0];
     double cs = dmax(C[s], 1.0e-100);
     lc[s] = log(cs);
     dlc[s] = dmax_dx(C[s], 1.0e-100) / cs;
-    g[s] = gibbs(tab + OFF_NASA + 17 * s, tT, lT);
+    g[s] = gibbs(tab + OFF_NASA + 17 * s, sp, tT, lT, iT);
   }
   // the diagonal only scales the error estimate: its accumulators and
   // factors are FP32 (a non-finite diagonal is caught by the caller)
@@ -383,7 +391,10 @@ __device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const
   }
 }
 
-// Solve A x = b (A overwritten) by LU with partial pivoting, all unrolled.
+// Solve A x = b (A overwritten) by LU with partial pivoting, all unrolled
+// (jnp.linalg.solve, hybrid/chemistry/integrator.py:81): the pivot swap is a
+// select over rows (no register array indexed at run time), each column's
+// multipliers use one reciprocal of its pivot.
 __device__ __forceinline__ void solve(double (*A)[NS], double* b) {
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
@@ -410,9 +421,10 @@ __device__ __forceinline__ void solve(double (*A)[NS], double* b) {
       b[k] = sw ? b[i] : t;
       b[i] = sw ? t : b[i];
     }
+    const double inv = 1.0 / A[k][k];
 #pragma unroll
     for (int i = k + 1; i < NS; ++i) {
-      double f = A[i][k] / A[k][k];
+      double f = A[i][k] * inv;
 #pragma unroll
       for (int j = k + 1; j < NS; ++j) A[i][j] -= f * A[k][j];
       b[i] -= f * b[k];
@@ -427,7 +439,10 @@ __device__ __forceinline__ void solve(double (*A)[NS], double* b) {
   }
 }
 
-// hybrid/chemistry/integrator.py:31-104
+// hybrid/chemistry/integrator.py:31-104: backward Euler on the species with
+// a fixed Newton count, non-negativity clip, temperature recovery at the
+// initial internal energy after each iteration (unless held), total-mass
+// rescale.  The reference evaluates its rates at (T, Tv) with Tv = T here.
 __device__ __forceinline__ double integrate(const double* tab, double* rho, double T, double dt, int iters,
                             int constant_temperature) {
   double rho0[NS], Y0[NS];
@@ -446,11 +461,11 @@ __device__ __forceinline__ double integrate(const double* tab, double* rho, doub
   double r[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) r[s] = rho0[s];
-  double Tk = T, Tvk = T;
+  double Tk = T;
   for (int it = 0; it < iters; ++it) {
     double w[NS];
     double J[NS][NS];
-    rates(tab, r, Tk, Tvk, w, J);
+    rates(tab, r, Tk, w, J);
     double rhs[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -461,10 +476,7 @@ __device__ __forceinline__ double integrate(const double* tab, double* rho, doub
     solve(J, rhs);
 #pragma unroll
     for (int s = 0; s < NS; ++s) r[s] = dmax(r[s] + rhs[s], 0.0);
-    if (!constant_temperature) {
-      Tk = recover_temperature(tab, r, e_init, Tk, 10);
-      Tvk = Tk;
-    }
+    if (!constant_temperature) Tk = recover_temperature_converged(tab, r, e_init, Tk, 10);
   }
   double sr = 0.0;
 #pragma unroll

@@ -6,7 +6,7 @@ the small-dt linearisation of the implicit step."""
 import numpy as np
 import pytest
 
-from oracle.chemistry import MOLAR_MASS, Mechanism, ReactingFlowCoupler, integrator_step
+from oracle.chemistry import MOLAR_MASS, R_UNIV, Mechanism, ReactingFlowCoupler, integrator_step
 
 MECH = Mechanism()
 
@@ -164,7 +164,26 @@ def test_device_table_transcription_matches_oracle():
     assert np.allclose(np.exp(der[:, 0]), MECH.A, rtol=1e-14, atol=0)
     assert np.array_equal(der[:, 1], d.sum(1))
     assert np.allclose(der[:, 2], d @ MOLAR_MASS, rtol=0, atol=1e-15)
-    assert o + 3 * nr == len(tab)
+    o += 3 * nr
+    # derived per species: 1/M, R/M, 1/t_low, 1/t_high, divided NASA-7
+    # coefficients per fit range; the Horner forms they feed equal the power
+    # sums of the fits
+    sp = tab[o:o + 18 * ns].reshape(ns, 18)
+    assert o + 18 * ns == len(tab)
+    assert np.allclose(sp[:, 0] * MOLAR_MASS, 1.0, rtol=1e-15, atol=0)
+    assert np.allclose(sp[:, 1] * MOLAR_MASS, R_UNIV, rtol=1e-15, atol=0)
+    for s, (lo, mid, hi, cl, ch) in enumerate(MECH.nasa):
+        assert sp[s, 2] * lo == pytest.approx(1.0, rel=1e-15)
+        assert sp[s, 3] * hi == pytest.approx(1.0, rel=1e-15)
+        for a, d7, t in ((cl, sp[s, 4:11], 0.5 * (lo + mid)), (ch, sp[s, 11:18], 0.5 * (mid + hi))):
+            h_power = (a[0] + a[1] * t / 2 + a[2] * t ** 2 / 3 + a[3] * t ** 3 / 4
+                       + a[4] * t ** 4 / 5 + a[5] / t)
+            h_horner = a[0] + t * (d7[0] + t * (d7[1] + t * (d7[2] + t * d7[3]))) + a[5] / t
+            s_power = (a[0] * np.log(t) + a[1] * t + a[2] * t ** 2 / 2 + a[3] * t ** 3 / 3
+                       + a[4] * t ** 4 / 4 + a[6])
+            s_horner = a[0] * np.log(t) + t * (a[1] + t * (d7[4] + t * (d7[5] + t * d7[6]))) + a[6]
+            assert h_horner == pytest.approx(h_power, rel=1e-12, abs=1e-12), s
+            assert s_horner == pytest.approx(s_power, rel=1e-12, abs=1e-12), s
 
 
 def test_be_check_pins():
