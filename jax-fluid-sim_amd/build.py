@@ -30,10 +30,18 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 # (their gradients are checked against the oracle).
 FAST_FP32_DIV = {"fsg_p1", "fsg_p2", "fsg_p3", "fsg_mood"}
 
+# The chemistry kernels write their multiply-adds as explicit fma() and are
+# built without automatic contraction, so that the coupler's arithmetic is the
+# same in every kernel that inlines it (the fused coupler and its split
+# prepare / integrate / finish path agree bit for bit).
+NO_CONTRACT = {"fsg_chem"}
+
 
 def _compile(src):
     obj = BUILD / (src.stem + ".o")
     extra = ["-fno-hip-fp32-correctly-rounded-divide-sqrt"] if src.stem in FAST_FP32_DIV else []
+    if src.stem in NO_CONTRACT:
+        extra = extra + ["-ffp-contract=off"]
     cmd = [HIPCC, *FLAGS, *extra, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

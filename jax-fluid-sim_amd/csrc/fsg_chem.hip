@@ -71,8 +71,8 @@ __device__ __forceinline__ double clamped_inverse(const double* c, const double*
 
 __device__ __forceinline__ void nasa_range(const double* a, const double* d, double t, double it,
                                            double& cp, double& h) {
-  cp = a[0] + t * (a[1] + t * (a[2] + t * (a[3] + t * a[4])));
-  h = a[0] + t * (d[0] + t * (d[1] + t * (d[2] + t * d[3]))) + a[5] * it;
+  cp = fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]);
+  h = fma(a[5], it, fma(t, fma(t, fma(t, fma(t, d[3], d[2]), d[1]), d[0]), a[0]));
 }
 
 __device__ __forceinline__ Thermo nasa(const double* c, const double* sp, double T, double iT) {
@@ -89,8 +89,8 @@ __device__ __forceinline__ Thermo nasa(const double* c, const double* sp, double
 // logarithm of the clamped temperature otherwise).
 __device__ __forceinline__ double gibbs_range(const double* a, const double* d, double t, double lt,
                                               double it) {
-  const double h = a[0] + t * (d[0] + t * (d[1] + t * (d[2] + t * d[3]))) + a[5] * it;
-  const double sr = a[0] * lt + t * (a[1] + t * (d[4] + t * (d[5] + t * d[6]))) + a[6];
+  const double h = fma(a[5], it, fma(t, fma(t, fma(t, fma(t, d[3], d[2]), d[1]), d[0]), a[0]));
+  const double sr = fma(a[0], lt, fma(t, fma(t, fma(t, fma(t, d[6], d[5]), d[4]), a[1]), a[6]));
   return h - sr;
 }
 
@@ -129,7 +129,7 @@ __device__ __forceinline__ double mixture_energy(const double* tab, const double
   for (int s = 0; s < NS; ++s) {
     const double* sp = species_block(tab, s);
     Thermo th = nasa(tab + OFF_NASA + 17 * s, sp, T, iT);
-    e += Y[s] * ((th.h - 1.0) * (sp[1] * T));
+    e = fma(Y[s], (th.h - 1.0) * (sp[1] * T), e);
   }
   return e;
 }
@@ -164,8 +164,8 @@ __device__ __forceinline__ double recover_temperature_converged(const double* ta
     for (int s = 0; s < NS; ++s) {
       const double* sp = species_block(tb, s);
       Thermo th = nasa(tb + OFF_NASA + 17 * s, sp, Ts, iTs);
-      e += Y[s] * ((th.h - 1.0) * (sp[1] * Ts));
-      cv += Y[s] * ((th.cp - 1.0) * sp[1]);
+      e = fma(Y[s], (th.h - 1.0) * (sp[1] * Ts), e);
+      cv = fma(Y[s], (th.cp - 1.0) * sp[1], cv);
     }
     double dT = -(e - e_target) / dmax(cv, 1.0e-5);
     const double Tn = clampd(T + clampd(dT, -0.5 * T, 0.5 * T), 100.0, 50000.0);
@@ -192,7 +192,8 @@ __device__ __forceinline__ double recover_temperature_converged(const double* ta
 __device__ __forceinline__ void rates(const double* __restrict__ tab, const double* rho, double T,
                                       double* w, double (*dw)[NS]) {
   const double* mw = tab;
-  double C[NS], dC[NS], lc[NS], dlc[NS], g[NS];
+  // C_s is formed again where a third body needs it (not held)
+  double dC[NS], lc[NS], dlc[NS], g[NS];
   const double tT = dmax(T, 10.0);
   const double lT = log(tT), iT = 1.0 / tT;
   const double lP = log(P_STD / (R_UNIV * tT));
@@ -200,11 +201,11 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
   for (int s = 0; s < NS; ++s) {
     const double* sp = species_block(tab, s);
     const double x = rho[s] * sp[0];
-    C[s] = dmax(x, 0.0);
+    const double C = dmax(x, 0.0);
     dC[s] = dmax_dx(x, 0.0) * sp[0];
-    const double cs = dmax(C[s], 1.0e-100);
+    const double cs = dmax(C, 1.0e-100);
     lc[s] = log(cs);
-    dlc[s] = dmax_dx(C[s], 1.0e-100) / cs;
+    dlc[s] = dmax_dx(C, 1.0e-100) / cs;
     g[s] = gibbs(tab + OFF_NASA + 17 * s, sp, tT, lT, iT);
   }
   double om[NS], csum[NS];
@@ -231,14 +232,15 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
     for (int s = 0; s < NS; ++s) {
       const double a1 = n1[s], a2 = n2[s];
       if (a1 != 0.0 || a2 != 0.0) {
-        dG += (a2 - a1) * g[s];
-        lf += a1 * lc[s];
-        lr += a2 * lc[s];
+        dG = fma(a2 - a1, g[s], dG);
+        lf = fma(a1, lc[s], lf);
+        lr = fma(a2, lc[s], lr);
       }
-      if (third && al[s] != 0.0) Meff += al[s] * C[s];
+      if (third && al[s] != 0.0)
+        Meff = fma(al[s], dmax(rho[s] * tb[OFF_SPEC + SPEC_W * s], 0.0), Meff);
     }
-    const double lkf = lnA + b * lT - th * iT;
-    const double lKc = dmax(-clampd(dG, -700.0, 700.0) + dn * lP, -690.7755278982137);
+    const double lkf = fma(-th, iT, fma(b, lT, lnA));
+    const double lKc = dmax(fma(dn, lP, -clampd(dG, -700.0, 700.0)), -690.7755278982137);
     const double fwdk = exp(lkf + clampd(lf, -700.0, 700.0));
     const double revk = exp(lkf - lKc + clampd(lr, -700.0, 700.0));
     const double cf = fabs(lf) < 700.0 ? 1.0 : (fabs(lf) == 700.0 ? 0.5 : 0.0);
@@ -247,24 +249,25 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
     const double net = fwdk - revk;
     const double q = net * col;
     const double fc = col * fwdk * cf, rc = col * revk * cr;
-    double dq[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const double d = n2[s] - n1[s];
+      if (d != 0.0) om[s] = fma(d, q, om[s]);
+    }
+    // column j of this reaction's d q / d rho, added to every row it feeds
+    // (no per-reaction vector held)
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       const double a1 = n1[j], a2 = n2[j];
       const double dcol = third ? al[j] : 0.0;
-      dq[j] = 0.0;
-      if (a1 != 0.0 || a2 != 0.0 || dcol != 0.0)
-        dq[j] = ((fc * a1 - rc * a2) * dlc[j] + dcol * net) * dC[j];
-      csum[j] += mr * dq[j];
-    }
+      if (a1 != 0.0 || a2 != 0.0 || dcol != 0.0) {
+        const double dq = fma(fma(fc, a1, -(rc * a2)), dlc[j], dcol * net) * dC[j];
+        csum[j] = fma(mr, dq, csum[j]);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const double d = n2[s] - n1[s];
-      if (d != 0.0) {
-        om[s] += d * q;
-        const double md = tb[s] * d;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) dw[s][j] += md * dq[j];
+        for (int s = 0; s < NS; ++s) {
+          const double d = n2[s] - n1[s];
+          if (d != 0.0) dw[s][j] = fma(tb[s] * d, dq, dw[s][j]);
+        }
       }
     }
   }
@@ -273,7 +276,7 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     srho += rho[s];
-    S += om[s] * mw[s];
+    S = fma(om[s], mw[s], S);
   }
   const double tot = dmax(srho, 1.0e-14), dtot = dmax_dx(srho, 1.0e-14);
   const double itot = 1.0 / tot;
@@ -282,14 +285,14 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
   for (int s = 0; s < NS; ++s) {
     const double rp = dmax(rho[s], 0.0);
     const double Y = rp * itot;
-    w[s] = om[s] * mw[s] - Y * S;
+    w[s] = fma(om[s], mw[s], -(Y * S));
     // d Y_s / d rho_j = delta_sj [rho_s > 0] / tot - rho_s dtot / tot^2
     const double off = rp * k2;
     const double diag = dmax_dx(rho[s], 0.0) * itot;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       const double dY = (s == j ? diag : 0.0) - off;
-      dw[s][j] = dw[s][j] - dY * S - Y * csum[j];
+      dw[s][j] = fma(-Y, csum[j], fma(-dY, S, dw[s][j]));
     }
   }
 }
@@ -344,15 +347,15 @@ __device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const
     for (int s = 0; s < NS; ++s) {
       const double a1 = n1[s], a2 = n2[s];
       if (a1 != 0.0 || a2 != 0.0) {
-        dG += (a2 - a1) * g[s];
-        lf += a1 * lc[s];
-        lr += a2 * lc[s];
+        dG = fma(a2 - a1, g[s], dG);
+        lf = fma(a1, lc[s], lf);
+        lr = fma(a2, lc[s], lr);
       }
-      if (third && al[s] != 0.0) Meff += al[s] * C[s];
+      if (third && al[s] != 0.0) Meff = fma(al[s], C[s], Meff);
     }
-    const double lkf = lnA + b * lT - th * iT;
+    const double lkf = fma(-th, iT, fma(b, lT, lnA));
     // ln K_c = -clamp(dG) + dn ln(P / R T); k_r = k_f / max(K_c, 1e-300)
-    const double lKc = dmax(-clampd(dG, -700.0, 700.0) + dn * lP, -690.7755278982137);
+    const double lKc = dmax(fma(dn, lP, -clampd(dG, -700.0, 700.0)), -690.7755278982137);
     const double fwdk = exp(lkf + clampd(lf, -700.0, 700.0));
     const double revk = exp(lkf - lKc + clampd(lr, -700.0, 700.0));
     const double cf = fabs(lf) < 700.0 ? 1.0 : (fabs(lf) == 700.0 ? 0.5 : 0.0);
@@ -364,12 +367,12 @@ __device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const
       const double a1 = n1[s], a2 = n2[s];
       const double dcol = third ? al[s] : 0.0;
       if (a1 != 0.0 || a2 != 0.0 || dcol != 0.0) {
-        const float dq = (float)((col * (fwdk * cf * a1 - revk * cr * a2) * dlc[s] +
-                                  dcol * (fwdk - revk)) * dC[s]);
+        const float dq = (float)(fma(col * fma(fwdk * cf, a1, -(revk * cr * a2)), dlc[s],
+                                     dcol * (fwdk - revk)) * dC[s]);
         const double d = a2 - a1;
-        om[s] += d * q;
-        dom[s] += (float)d * dq;
-        csum[s] += (float)mr * dq;
+        om[s] = fma(d, q, om[s]);
+        dom[s] = fmaf((float)d, dq, dom[s]);
+        csum[s] = fmaf((float)mr, dq, csum[s]);
       }
     }
   }
@@ -426,15 +429,15 @@ __device__ __forceinline__ void solve(double (*A)[NS], double* b) {
     for (int i = k + 1; i < NS; ++i) {
       double f = A[i][k] * inv;
 #pragma unroll
-      for (int j = k + 1; j < NS; ++j) A[i][j] -= f * A[k][j];
-      b[i] -= f * b[k];
+      for (int j = k + 1; j < NS; ++j) A[i][j] = fma(-f, A[k][j], A[i][j]);
+      b[i] = fma(-f, b[k], b[i]);
     }
   }
 #pragma unroll
   for (int k = NS - 1; k >= 0; --k) {
     double acc = b[k];
 #pragma unroll
-    for (int j = k + 1; j < NS; ++j) acc -= A[k][j] * b[j];
+    for (int j = k + 1; j < NS; ++j) acc = fma(-A[k][j], b[j], acc);
     b[k] = acc / A[k][k];
   }
 }
@@ -469,9 +472,9 @@ __device__ __forceinline__ double integrate(const double* tab, double* rho, doub
     double rhs[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      rhs[s] = -(r[s] - rho0[s] - dt * w[s]);
+      rhs[s] = -fma(-dt, w[s], r[s] - rho0[s]);
 #pragma unroll
-      for (int j = 0; j < NS; ++j) J[s][j] = (s == j ? 1.0 : 0.0) - dt * J[s][j];
+      for (int j = 0; j < NS; ++j) J[s][j] = fma(-dt, J[s][j], s == j ? 1.0 : 0.0);
     }
     solve(J, rhs);
 #pragma unroll
@@ -526,21 +529,21 @@ __global__ __launch_bounds__(64) void react_kernel(const double* __restrict__ ta
 #pragma unroll
   for (int s = 0; s < NS; ++s) rs[s] *= scale;
   double ek = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / (rt * rt);
-  if (p.dimensionless) {
-    double eth_s = dmax(u[4] / rt - ek, 1.0e-4);
-    double eth = eth_s * p.e_ref;
-    double rp[NS];
+  // the dimensionless and dimensional couplers differ in their scalings
+  // only: one inlined integrator serves both
+  const bool dl = p.dimensionless;
+  const double eth_s = dmax(u[4] / rt - ek, dl ? 1.0e-4 : 1.0e3);
+  const double eth = dl ? eth_s * p.e_ref : eth_s;
+  if (dl) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) rp[s] = rs[s] * p.rho_ref;
-    double Tg = dmax((p.gamma - 1.0) * eth_s * p.T_ref, 200.0);
-    double T0 = recover_temperature_converged(tab, rp, eth, Tg, 8);
-    integrate(tab, rp, T0, h * p.t_ref, p.newton_iters, 0);
+    for (int s = 0; s < NS; ++s) rs[s] = rs[s] * p.rho_ref;
+  }
+  const double Tg = dl ? dmax((p.gamma - 1.0) * eth_s * p.T_ref, 200.0) : 1000.0;
+  const double T0 = recover_temperature_converged(tab, rs, eth, Tg, 8);
+  integrate(tab, rs, T0, dl ? h * p.t_ref : h, p.newton_iters, 0);
+  if (dl) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) rs[s] = rp[s] / p.rho_ref;
-  } else {
-    double eth = dmax(u[4] / rt - ek, 1.0e3);
-    double T0 = recover_temperature_converged(tab, rs, eth, 1000.0, 8);
-    integrate(tab, rs, T0, h, p.newton_iters, 0);
+    for (int s = 0; s < NS; ++s) rs[s] = rs[s] / p.rho_ref;
   }
   double s2 = 0.0;
 #pragma unroll
