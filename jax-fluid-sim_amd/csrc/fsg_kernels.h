@@ -431,21 +431,36 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
     real acc[V], acc_a = real(0);
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = real(0);
+    // The operator entries D[q][r][i] of a chunk of CH points are loaded
+    // together (one memory round trip per chunk instead of one per point).
+    constexpr int NQ = (Q + DS - 1) / DS, CH = NQ < 8 ? NQ : 8;
     if (i < N)
-      for (int q = h; q < Q; q += DS) {
-        const real* f = sF + q * SW;
-        real d[3];
+#pragma unroll 1
+      for (int k0 = 0; k0 < NQ; k0 += CH) {
+        real dc[CH][3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          d[r] = a.D[(q * 3 + r) * N + i];
+        for (int c = 0; c < CH; ++c) {
+          const int q = h + (k0 + c) * DS;
 #pragma unroll
-          for (int v = 0; v < VF; ++v) acc[v] += d[r] * f[r * VF + v];
+          for (int r = 0; r < 3; ++r) dc[c][r] = q < Q ? a.D[(q * 3 + r) * N + i] : real(0);
         }
-        if constexpr (V > VF) {
-          const real c = d[0] * f[15] + d[1] * f[16] + d[2] * f[17];
-          acc_a += d[0] * f[18] + d[1] * f[19] + d[2] * f[20];
 #pragma unroll
-          for (int v = VF; v < V; ++v) acc[v] += c * f[21 + (v - VF)];
+        for (int c = 0; c < CH; ++c) {
+          const int q = h + (k0 + c) * DS;
+          if (q >= Q) break;
+          const real* f = sF + q * SW;
+          const real* d = dc[c];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+#pragma unroll
+            for (int v = 0; v < VF; ++v) acc[v] += d[r] * f[r * VF + v];
+          }
+          if constexpr (V > VF) {
+            const real cc = d[0] * f[15] + d[1] * f[16] + d[2] * f[17];
+            acc_a += d[0] * f[18] + d[1] * f[19] + d[2] * f[20];
+#pragma unroll
+            for (int v = VF; v < V; ++v) acc[v] += cc * f[21 + (v - VF)];
+          }
         }
       }
 #pragma unroll
