@@ -186,6 +186,7 @@ int run_elements(fsg_context* c, const void* const* U, const void* const* base, 
     a.terms = terms;
     a.diag = reinterpret_cast<StageDiag*>(diag);
     a.mask = mask ? mask[b] : nullptr;
+    a.count = d.count;
     long ev = timing_begin(c, int(c->groups.size() + b), s, mask != nullptr);
     const int mfma = c->element_mfma > 0 && !wale ? 1 : 0;
     int rc = launch_element<P, V, real>(d.element_type, d.count, a, ph, s, mfma);

@@ -37,8 +37,11 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
     if constexpr (P >= 2) {                                                                     \
       if (type == HEX && a.sf1d) {                                                              \
         constexpr int NTS = P == 3 ? 256 : 64;                                                  \
-        hipLaunchKernelGGL((hex_sf_kernel<P, V, real, NTS>), dim3((unsigned)count), dim3(NTS), 0, \
-                           s, a, ph);                                                           \
+        const dim3 gs((unsigned)pass_grid(count, a.mask != nullptr));                           \
+        if (a.mask)                                                                             \
+          hipLaunchKernelGGL((hex_sf_kernel<P, V, real, NTS, true>), gs, dim3(NTS), 0, s, a, ph); \
+        else                                                                                    \
+          hipLaunchKernelGGL((hex_sf_kernel<P, V, real, NTS, false>), gs, dim3(NTS), 0, s, a, ph); \
         return hipGetLastError() == hipSuccess ? 0 : -1;                                        \
       }                                                                                         \
     }                                                                                           \
@@ -57,7 +60,7 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
   int launch_face<P, V, real>(int ft, int to, int tn, int sub, int64_t count,                   \
                               const FaceArgs<real>& a, const Phys<real>& ph, hipStream_t s) {   \
     if (count <= 0) return 0;                                                                   \
-    dim3 grid((unsigned)count), block(64);                                                      \
+    dim3 grid((unsigned)pass_grid(count, a.mask_o != nullptr)), block(64);                      \
     const int key = ((ft * 2 + sub) * 4 + to) * 4 + (tn + 1);                                   \
     switch (key) {                                                                              \
       FSG_FACE_CASE(QUAD, 0, HEX, HEX, P, V, real)                                              \
@@ -89,14 +92,24 @@ int launch_face(int face_type, int owner_type, int neighbor_type, int sub, int64
     break;                                                                                      \
   }
 
+// WALE and the restricted pass are separate instantiations (the plain full
+// pass keeps its registers).
 #define FSG_FACE_CASE(FT, SUB, TO, TN, P, V, real)                                             \
   case ((FT * 2 + SUB) * 4 + TO) * 4 + (TN + 1):                                               \
-    if (ph.wale)                                                                                \
-      hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, true>), grid, block, 0, s, a, \
-                         ph);                                                                   \
-    else                                                                                        \
-      hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, false>), grid, block, 0, s,  \
-                         a, ph);                                                                \
+    if (a.mask_o) {                                                                             \
+      if (ph.wale)                                                                              \
+        hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, true, true>), grid, block, \
+                           0, s, a, ph);                                                        \
+      else                                                                                      \
+        hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, false, true>), grid, block, \
+                           0, s, a, ph);                                                        \
+    } else if (ph.wale) {                                                                       \
+      hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, true, false>), grid, block, 0, \
+                         s, a, ph);                                                             \
+    } else {                                                                                    \
+      hipLaunchKernelGGL((face_kernel<FT, TO, TN, SUB, P, V, real, false, false>), grid, block, \
+                         0, s, a, ph);                                                          \
+    }                                                                                           \
     break;
 
 }  // namespace fsg

@@ -78,7 +78,8 @@ __device__ __forceinline__ void pencil_vals(const real* op, const real* in, int 
 }
 
 template <int P, int V, typename real, int NT>
-__global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real> ph) {
+__device__ __forceinline__ void hex_sf_body(const ElemArgs<real>& a, const Phys<real>& ph,
+                                            const int64_t e) {
   using S = HexSF<P>;
   constexpr int N1 = S::N1, Q1 = S::Q1, N = S::N, Q = S::Q;
   constexpr int W = V + 12;
@@ -113,8 +114,6 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
   __shared__ unsigned int redn[NT / 64];
 
   const int tid = threadIdx.x;
-  const int64_t e = blockIdx.x;
-  if (a.mask && a.mask[e] == 0) return;   // whole workgroup: no barrier passed yet
   DiagSeen seen{};
   if (a.mode == 2 && tid == 0) seen = diag_seen(a.diag);
   const bool euler = a.terms & TERM_EULER;
@@ -433,6 +432,12 @@ __global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real>
     atomic_min_nan<real>(&a.diag->p_min, pmn, seen.p);
     if (bad) atomicAdd(&a.diag->nonfinite, bad);
   }
+}
+
+template <int P, int V, typename real, int NT, bool RESTRICTED>
+__global__ __launch_bounds__(NT) void hex_sf_kernel(ElemArgs<real> a, Phys<real> ph) {
+  for_each_item<RESTRICTED>(a.count, [&](int64_t e) { return a.mask[e] != 0; },
+                            [&](int64_t e) { hex_sf_body<P, V, real, NT>(a, ph, e); });
 }
 
 }  // namespace fsg

@@ -58,6 +58,7 @@ struct ElemArgs {
   int terms;
   StageDiag* diag;
   const int32_t* mask;    // (E) evaluate only elements with mask != 0, or null: all
+  int64_t count;          // elements of the block
 };
 
 template <typename real>
@@ -138,6 +139,45 @@ __host__ __device__ constexpr int div_split(int N, int NT) {
   int d = 1;
   while (d < 64 && N * d * 2 <= NT) d *= 2;
   return d;
+}
+
+// The items (elements or faces) a workgroup evaluates.  A full pass has one
+// item per workgroup, blockIdx.x.  A restricted pass (a mask is given: the
+// viscous rhs of MOOD-troubled elements only; a separate instantiation, so
+// that the full pass keeps its registers) runs on a capped grid that
+// strides over chunks of 64 items: each lane tests one item, a ballot
+// collects the marked ones and the workgroup evaluates those in turn.  With
+// nothing marked such a pass costs one mask read per item instead of one
+// workgroup launch per item.  Every wave of a workgroup forms the same
+// ballot, so the items are uniform across the workgroup (its barriers are
+// reached by all of its waves).
+constexpr int64_t RESTRICTED_GRID = 2048;
+
+__host__ __device__ constexpr int64_t pass_grid(int64_t count, bool restricted) {
+  return restricted ? ((count + 63) / 64 < RESTRICTED_GRID ? (count + 63) / 64 : RESTRICTED_GRID)
+                    : count;
+}
+
+template <bool RESTRICTED, class Marked, class Body>
+__device__ __forceinline__ void for_each_item(int64_t count, Marked marked, Body body) {
+  if constexpr (!RESTRICTED) {
+    body((int64_t)blockIdx.x);
+  } else {
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * 64;
+    for (int64_t c0 = (int64_t)blockIdx.x * 64; c0 < count; c0 += stride) {
+      const int64_t i = c0 + lane;
+      uint64_t bits = __ballot(i < count && marked(i));
+      while (bits) {
+        const int j = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        body(c0 + j);
+        // the next item's staging must not overwrite LDS that a slower wave
+        // still reads at the end of this one
+        __syncthreads();
+      }
+    }
+  }
 }
 
 __device__ __forceinline__ int64_t readlane64(int64_t x, int j) {
@@ -229,6 +269,10 @@ __device__ __forceinline__ void add_face_records(const ElemArgs<real>& a, int64_
   }
 }
 
+// The dense element kernels keep one element per workgroup also in a
+// restricted pass (an early exit on unmarked elements): a restricted
+// instantiation of their body changed the full pass's register allocation
+// (119 -> 130 VGPRs for tetrahedra in FP32, below four waves per SIMD).
 template <int T, int P, int V, typename real, int NT>
 __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real> ph) {
   constexpr int N = n_nodes(T, P);
@@ -648,7 +692,8 @@ __device__ __forceinline__ void face_grad_hex_sf(const int32_t* __restrict__ met
 // gradient (a separate instantiation, so that the plain kernels keep their
 // registers).
 template <int FT, int TO, int TN, int SUB, int P, int V, typename real, bool WALE>
-__global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> ph) {
+__device__ __forceinline__ void face_body(const FaceArgs<real>& a, const Phys<real>& ph,
+                                          const int64_t f) {
   constexpr int NO = n_nodes(TO, P);
   constexpr int NN = TN >= 0 ? n_nodes(TN, P) : 1;
   constexpr int QF = n_face_q(FT, P);
@@ -693,16 +738,12 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
   real* const sfs = sAA;                 // after phase 1b
 
   const int tid = threadIdx.x;
-  const int64_t f = blockIdx.x;
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;
 
   const int64_t eo = a.owner_elem[f];
   const int64_t en = NBR ? a.neighbor_elem[f] : 0;
-  // restricted evaluation: only faces with a marked side (one face per
-  // workgroup, so the whole workgroup leaves before any barrier)
-  if (a.mask_o && a.mask_o[eo] == 0 && !(NBR && a.mask_n[en] != 0)) return;
   const real n[3] = {a.geom[f * 4 + 0], a.geom[f * 4 + 1], a.geom[f * 4 + 2]};
   const real area = a.geom[f * 4 + 3];
   const int flags = NBR ? 0 : a.bflags[f];
@@ -1277,6 +1318,18 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> p
         for (int k = tid; k < FN * 12; k += 64) on[(k / 12) * W + V + (k % 12)] = sbn[k];
     }
   }
+}
+
+template <int FT, int TO, int TN, int SUB, int P, int V, typename real, bool WALE, bool RESTRICTED>
+__global__ __launch_bounds__(64) void face_kernel(FaceArgs<real> a, Phys<real> ph) {
+  constexpr bool NBR = TN >= 0;
+  // a restricted pass evaluates the faces with a marked side
+  for_each_item<RESTRICTED>(
+      a.count,
+      [&](int64_t f) {
+        return a.mask_o[a.owner_elem[f]] != 0 || (NBR && a.mask_n[a.neighbor_elem[f]] != 0);
+      },
+      [&](int64_t f) { face_body<FT, TO, TN, SUB, P, V, real, WALE>(a, ph, f); });
 }
 
 }  // namespace fsg

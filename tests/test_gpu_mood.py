@@ -255,8 +255,11 @@ def test_fp32_path(name, p):
 @pytest.mark.parametrize("name,p", [("split", 2), ("split", 3)])
 def test_masked_viscous_rhs_is_bitwise(name, p, dtype):
     """The viscous rhs restricted to marked elements equals the full viscous
-    rhs bit for bit on those elements, on every element path (the masked
-    pass re-evaluates every face of a marked element)."""
+    rhs on those elements, on every element path (the masked pass
+    re-evaluates every face of a marked element): bit for bit in FP64; in
+    FP32 within 1e-6 of the block's largest value (the restricted pass is a
+    separate kernel instantiation, a loop over the marked elements, and the
+    compiler may contract its FP32 multiply-adds differently)."""
     print(f"case {name} p={p} {dtype} masked viscous rhs", flush=True)
     solver, op, b, lim, oop, ob, olim, states = _case(name, p, dtype=dtype)
     U = _dev(solver, states)
@@ -274,7 +277,14 @@ def test_masked_viscous_rhs_is_bitwise(name, p, dtype):
             for t in op.block_types:
                 m = masks[t].bool()
                 assert int(m.sum()) > 0
-                assert torch.equal(part[t][m], full[t][m]), (t, mfma, sf)
+                if dtype == torch.float64:
+                    assert torch.equal(part[t][m], full[t][m]), (t, mfma, sf)
+                else:
+                    d = float((part[t][m] - full[t][m]).abs().max())
+                    ref = float(full[t].abs().max())
+                    print(f"{t} mfma={mfma} sf={sf}: max |diff| / max |rhs| {d / ref:.2e}",
+                          flush=True)
+                    assert d <= 1e-6 * ref, (t, mfma, sf, d, ref)
     finally:
         dm.set_element_path(None)
         dm.set_hex_sum_factorisation(None)
