@@ -19,6 +19,7 @@
 //     hybrid/time_integration.py:264-312, 404-439.
 #pragma once
 #include "fsg_lanes.h"
+#include "fsg_mfma.h"
 #include "fsg_physics.h"
 
 namespace fsg {
@@ -842,9 +843,49 @@ __device__ __forceinline__ void face_body(const FaceArgs<real>& a, const Phys<re
     sB[tid][5] = wa.vel[0] * n[0] + wa.vel[1] * n[1] + wa.vel[2] * n[2];
   }
 
-  // Phase 1a: traces at the face points, one lane per (point, variable):
-  // primal Up = sum_k phi_k u_k (for the viscous flux) and anchored
-  // Ua = u_0 + sum_k phi_k (u_k - u_0) (for the Euler flux).
+  // Phase 1a: traces at the face points: primal Up = sum_k phi_k u_k (for
+  // the viscous flux) and anchored Ua = u_0 + sum_k phi_k (u_k - u_0) (for
+  // the Euler flux).  With at most 16 face points, 16 support nodes and 16
+  // variables each is a 16x16 tile product Phi (points x nodes) U (nodes x
+  // variables) on the matrix cores (v_mfma 16x16x4, K = support nodes in
+  // steps of 4; rows and columns past the sizes are zero); otherwise one lane
+  // per (point, variable).  FP32 quadrilateral faces keep the vector form:
+  // there the tile accumulators cost two waves per SIMD of occupancy for no
+  // gain (hex-hex p=2: 1.37 ms either way), while the FP64 kernels (hex-hex
+  // 3.32 -> 2.79 ms) and the FP32 triangles (hex-tet 1.08 -> 1.02 ms) gain.
+  constexpr bool MT = QF <= 16 && FO <= 16 && FN <= 16 && V <= 16 &&
+                      !(sizeof(real) == 4 && FT == QUAD);
+  if constexpr (MT) {
+    const int lr = tid & 15, lk = tid >> 4;
+    auto traces = [&](const real* tb, const real* sUs, int NF, real* sA, real* sP) {
+      const real u0 = lr < V ? sUs[lr] : real(0);
+      auto ca = zero4<real>(), cp = zero4<real>();
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4) {
+        if (k0 >= NF) break;
+        const int k = k0 + lk;
+        const bool in = k < NF;
+        const real phi = (in && lr < QF) ? tb[lr * NF + k] : real(0);
+        const real u = (in && lr < V) ? sUs[(k + 1) * V + lr] : real(0);
+        ca = mma(phi, in ? u - u0 : real(0), ca);
+        cp = mma(phi, u, cp);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int q = crow<real>(tid, j);
+        if (q < QF && lr < V) {
+          sA[q * V + lr] = ca[j] + u0;
+          sP[q * V + lr] = cp[j];
+        }
+      }
+    };
+    traces(tbo, sUo, FO, sAo, sPo);
+    if constexpr (NBR) {
+      traces(tbn, sUn, FN, sAn, sPn);
+    } else {
+      for (int k = tid; k < QF * V; k += 64) sPn[k] = sPo[k];  // unused: the boundary flux is the owner's
+    }
+  } else
   for (int k = tid; k < QF * V; k += 64) {
     const int q = k / V, v = k - q * V;
     const real* phio = tbo + q * FO;
@@ -1274,8 +1315,41 @@ __device__ __forceinline__ void face_body(const FaceArgs<real>& a, const Phys<re
     __syncthreads();
   }
 
-  // Phase 5: project onto the face-support basis of each side: four lanes per
+  // Phase 5: project onto the face-support basis of each side, records
+  // rec[k][v] = sum_q phi_k(x_q) S_q,v: with the tile sizes of phase 1a a
+  // 16x16 tile product Phi^T (nodes x points) S (points x variables) on the
+  // matrix cores, written straight to the records; otherwise four lanes per
   // support node split the points.
+  if constexpr (MT) {
+    const int lr = tid & 15, lk = tid >> 4;
+    auto project = [&](const real* tb, const real* sS, int NF, real* out) {
+      auto c = zero4<real>();
+#pragma unroll
+      for (int q0 = 0; q0 < 16; q0 += 4) {
+        if (q0 >= QF) break;
+        const int q = q0 + lk;
+        const bool in = q < QF;
+        const real phi = (in && lr < NF) ? tb[q * NF + lr] : real(0);
+        const real sv = (in && lr < V) ? sS[q * V + lr] : real(0);
+        c = mma(phi, sv, c);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int kk = crow<real>(tid, j);
+        if (kk < NF && lr < V) out[kk * W + lr] = c[j];
+      }
+    };
+    real* oo = a.out_o + f * (FO * W);
+    project(tbo, sSo, FO, oo);
+    if (visc)
+      for (int k = tid; k < FO * 12; k += 64) oo[(k / 12) * W + V + (k % 12)] = sbo[k];
+    if constexpr (NBR) {
+      real* on = a.out_n + f * (FN * W);
+      project(tbn, sSn, FN, on);
+      if (visc)
+        for (int k = tid; k < FN * 12; k += 64) on[(k / 12) * W + V + (k % 12)] = sbn[k];
+    }
+  } else
   {
     const int kk = tid >> 2, part = tid & 3;
     real ao[V], an[V];
