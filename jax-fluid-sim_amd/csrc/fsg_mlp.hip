@@ -171,6 +171,7 @@ constexpr int HID = 128;
 constexpr int NIN = 14;
 constexpr int NOUT = 11;
 constexpr int PROJ_ITERS = 32;   // damped dual Newton iterations, at most
+constexpr int PROJ_ITERS_F32 = 8;  // of which first in FP32, at most
 // Packed bf16 weights (A-operand order, see pack_weights in the host code):
 //   W1: 8 out-tiles x 1 k-step, W2/W3: 8 x 4, W4: 1 x 4 ; each (tile, step)
 //   block holds 64 lanes x 8 bf16.
@@ -419,7 +420,70 @@ __global__ __launch_bounds__(64) void project_kernel(const InferConsts cstv, con
       // Newton on the dual with the step limited to a change of 2 in any
       // species' log (the electron column of A is ~1e6 times the others, so
       // a full step from lam = 0 can overflow exp); stops when converged.
+      // The first iterations run in FP32 (hardware exponentials) until the
+      // step falls below 1e-4; the FP64 iterations then start from there and
+      // converge to the same stop (a step of at most 1e-14), in one or two
+      // iterations instead of four or five from lam = 0.  A non-finite FP32
+      // result is discarded (the FP64 iterations then start from 0).
       double lam[3] = {0, 0, 0};
+      {
+        float lf[3] = {0.f, 0.f, 0.f};
+        for (int it = 0; it < PROJ_ITERS_F32; ++it) {
+          float Yq[NOUT];
+#pragma unroll
+          for (int s = 0; s < NOUT; ++s)
+            Yq[s] = (float)Yp[s] * __expf((float)A[0][s] * lf[0] + (float)A[1][s] * lf[1] +
+                                          (float)A[2][s] * lf[2]);
+          float rr[3], J[3][3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            float acc = 0.f;
+#pragma unroll
+            for (int s = 0; s < NOUT; ++s) acc += (float)A[i][s] * Yq[s];
+            rr[i] = acc - (float)cvec[i];
+#pragma unroll
+            for (int k = i; k < 3; ++k) {
+              float jk = 0.f;
+#pragma unroll
+              for (int s = 0; s < NOUT; ++s) jk += (float)A[i][s] * Yq[s] * (float)A[k][s];
+              J[i][k] = jk;
+              J[k][i] = jk;
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+#pragma unroll
+            for (int i = k + 1; i < 3; ++i) {
+              const float f = J[i][k] / J[k][k];
+#pragma unroll
+              for (int j = k; j < 3; ++j) J[i][j] -= f * J[k][j];
+              rr[i] -= f * rr[k];
+            }
+          }
+          float d[3];
+#pragma unroll
+          for (int k = 2; k >= 0; --k) {
+            float acc = rr[k];
+#pragma unroll
+            for (int j = k + 1; j < 3; ++j) acc -= J[k][j] * d[j];
+            d[k] = acc / J[k][k];
+          }
+          float step = 0.f;
+#pragma unroll
+          for (int s = 0; s < NOUT; ++s) {
+            const float e = fabsf((float)A[0][s] * d[0] + (float)A[1][s] * d[1] +
+                                  (float)A[2][s] * d[2]);
+            step = e > step ? e : step;
+          }
+          const float damp = step > 2.f ? 2.f / step : 1.f;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) lf[k] -= damp * d[k];
+          if (!(step > 1e-4f)) break;
+        }
+        const bool fin = (lf[0] - lf[0] == 0.f) && (lf[1] - lf[1] == 0.f) && (lf[2] - lf[2] == 0.f);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lam[k] = fin ? (double)lf[k] : 0.0;
+      }
       double res = 0.0;
       for (int it = 0; it < PROJ_ITERS; ++it) {
         double Yq[NOUT];
