@@ -169,8 +169,9 @@ int fsg_residual(fsg_context* ctx, const void* const* U, void* const* out, int32
 /* fsg_residual restricted to marked elements: mask[b] is an (E_b) int32
  * array per block; the faces with a marked side and the marked elements are
  * evaluated, and out[b] holds the result on the elements with mask != 0
- * (bitwise equal to fsg_residual there; other elements' values are
- * unspecified).  Serves the MOOD stage, whose viscous rhs is read only on
+ * (equal to fsg_residual there: bitwise in FP64, to rounding in FP32, where
+ * the restricted kernels may contract multiply-adds differently; other
+ * elements' values are unspecified).  Serves the MOOD stage, whose viscous rhs is read only on
  * the troubled elements (hybrid/subcell.py:876-886). */
 int fsg_residual_masked(fsg_context* ctx, const void* const* U, void* const* out, int32_t mode,
                         int32_t terms, const int32_t* const* mask, const fsg_boundary* boundary,
@@ -181,7 +182,9 @@ int fsg_residual_masked(fsg_context* ctx, const void* const* U, void* const* out
  *   next = alpha * base + beta * (curr + dt * rhs(curr)),
  * followed by the node admissibility reductions into diag (device).  dt is
  * a scalar unless dt_elem is non-null (one dt per element, the LTS dict).
- * next may alias curr or base. */
+ * next may alias curr or base.  base may be null (no base term: next =
+ * beta * (curr + dt * rhs), the Euler candidate of a MOOD stage with
+ * alpha = 0, beta = 1) and diag may be null (no admissibility reductions). */
 int fsg_rk_stage(fsg_context* ctx, const void* const* base, const void* const* curr,
                  void* const* next, double dt, const void* const* dt_elem, double alpha,
                  double beta, int32_t terms, const fsg_boundary* boundary,
@@ -444,7 +447,10 @@ int fsg_mood_stage(fsg_mood* m, const void* const* base, const void* const* curr
  * only (fsg_residual_masked, into visc, which must hold (E_b, N, nvars)
  * arrays; null when the viscosity is zero) and feeds their subcell
  * recompute.  Same stage as hybrid/time_integration.py:404-439 up to the
- * rounding of inv + visc. */
+ * rounding of inv + visc.  total may be null: work then already holds the
+ * Euler candidate curr + dt * rhs(curr) (fsg_rk_stage with a null base,
+ * alpha 0, beta 1, the same dt), which detection reads instead of total and
+ * curr. */
 int fsg_mood_stage_fused(fsg_mood* m, const void* const* base, const void* const* curr,
                          const void* const* total, void* const* visc, void* const* work,
                          void* const* next, double dt, const void* const* dt_elem, double alpha,

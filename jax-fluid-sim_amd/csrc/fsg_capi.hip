@@ -295,7 +295,9 @@ int fsg_rk_stage(fsg_context* ctx, const void* const* base, const void* const* c
                  void* const* next, double dt, const void* const* dt_elem, double alpha,
                  double beta, int32_t terms, const fsg_boundary* boundary, fsg_stage_diag* diag,
                  void* stream) {
-  if (!ctx || !base || !curr || !next || !diag) return fail(-10, "null argument");
+  // base may be null (next = beta (curr + dt rhs), no base term read) and
+  // diag may be null (no admissibility reduction)
+  if (!ctx || !curr || !next) return fail(-10, "null argument");
   return dispatch(ctx, curr, base, next, 2, terms, dt, dt_elem, alpha, beta, boundary, diag,
                   nullptr, static_cast<hipStream_t>(stream));
 }

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t e0 = (int64_t)blockIdx.x * EB;
   DiagSeen seen{};
-  if (a.mode == 2 && lane == 0) seen = diag_seen(a.diag);
+  if (a.mode == 2 && a.diag && lane == 0) seen = diag_seen(a.diag);
   const int ne = (int)(count - e0 < EB ? count - e0 : EB);
   if (a.mask) {
     // masked evaluation: a batch without a marked element has nothing to do
@@ -411,14 +411,15 @@ __global__ __launch_bounds__(256) void element_mfma_kernel(ElemArgs<real> a, Phy
           if (!a.mask || a.mask[e0 + e] != 0) a.out[gi] = rhs;
         } else {
           const real dte = a.dt_elem ? a.dt_elem[e0 + e] : a.dt;
-          const real nxt = a.alpha * a.Ubase[gi] + a.beta * (sU[i * WU + col] + dte * rhs);
+          const real nxt = (a.Ubase ? a.alpha * a.Ubase[gi] : real(0)) +
+                           a.beta * (sU[i * WU + col] + dte * rhs);
           a.out[gi] = nxt;
           sR[i * WU + col] = nxt;
         }
       }
     }
   }
-  if (a.mode != 2) return;
+  if (a.mode != 2 || !a.diag) return;
   __syncthreads();
 
   // Admissibility at the nodes (hybrid/time_integration.py:291-312).

@@ -115,7 +115,7 @@ __device__ __forceinline__ void hex_sf_body(const ElemArgs<real>& a, const Phys<
 
   const int tid = threadIdx.x;
   DiagSeen seen{};
-  if (a.mode == 2 && tid == 0) seen = diag_seen(a.diag);
+  if (a.mode == 2 && a.diag && tid == 0) seen = diag_seen(a.diag);
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;
@@ -390,12 +390,13 @@ __device__ __forceinline__ void hex_sf_body(const ElemArgs<real>& a, const Phys<
     if (a.mode == 1) {
       out[k] = rhs;
     } else {
-      const real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (Ue[k] + dte * rhs);
+      const real nxt = (a.Ubase ? a.alpha * a.Ubase[e * (N * V) + k] : real(0)) +
+                       a.beta * (Ue[k] + dte * rhs);
       out[k] = nxt;
       sR[k] = nxt;
     }
   }
-  if (a.mode != 2) return;
+  if (a.mode != 2 || !a.diag) return;
   __syncthreads();
 
   // Admissibility at nodes (hybrid/time_integration.py:291-312).

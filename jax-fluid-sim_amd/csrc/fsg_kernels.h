@@ -298,7 +298,7 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
   const int64_t e = blockIdx.x;
   if (a.mask && a.mask[e] == 0) return;   // whole workgroup: no barrier passed yet
   DiagSeen seen{};
-  if (a.mode == 2 && tid == 0) seen = diag_seen(a.diag);
+  if (a.mode == 2 && a.diag && tid == 0) seen = diag_seen(a.diag);
   const bool euler = a.terms & TERM_EULER;
   const bool visc = a.terms & TERM_VISCOUS;
   const real gamma = ph.gamma;
@@ -534,7 +534,10 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
     if (a.mode == 1) {
       out[k] = rhs;
     } else {
-      const real nxt = a.alpha * a.Ubase[e * (N * V) + k] + a.beta * (sU[k] + dte * rhs);
+      // without a base state (the Euler candidate U + dt rhs of a MOOD stage)
+      // no base term is read
+      const real nxt = (a.Ubase ? a.alpha * a.Ubase[e * (N * V) + k] : real(0)) +
+                       a.beta * (sU[k] + dte * rhs);
       out[k] = nxt;
       sF[k] = nxt;
     }
@@ -560,7 +563,7 @@ __global__ __launch_bounds__(NT) void element_kernel(ElemArgs<real> a, Phys<real
       for (int v = 0; v < V; ++v) finish(i * V + v, acc[v]);
     }
   }
-  if (a.mode != 2) return;
+  if (a.mode != 2 || !a.diag) return;
   __syncthreads();
 
   // Admissibility at nodes (hybrid/time_integration.py:291-312).

@@ -391,7 +391,28 @@ __global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph
     real* su = &sU[w][0][0];
     const int64_t o = e * N * V;
     const real mwl = lane < N ? T.mw[lane] : real(0);
-    if (a.stage) {
+    if (a.stage && !a.inv[b]) {
+      // work already holds the Euler candidate curr + dt rhs (formed by the
+      // residual's own epilogue): the stage candidate is alpha base + beta work
+      const real* wrk = a.work[b] + o;
+      const real* bas = a.base[b] + o;
+      constexpr int CH = 4;
+      const int n = N * V;
+      for (int k0 = 0; k0 < n; k0 += LG * CH) {
+        real rw[CH], rb[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int k = k0 + lane + LG * i, kk = k < n ? k : n - 1;
+          rw[i] = wrk[kk];
+          rb[i] = bas[kk];
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int k = k0 + lane + LG * i;
+          if (k < n) su[k] = a.alpha * rb[i] + a.beta * rw[i];
+        }
+      }
+    } else if (a.stage) {
       const real h = a.dte[b] ? a.dte[b][e] : a.dt;
       const real* inv = a.inv[b] + o;
       const real* vis = a.visc[b] ? a.visc[b] + o : nullptr;
@@ -1735,7 +1756,7 @@ int fsg_mood_stage_fused(fsg_mood* m, const void* const* base, const void* const
                          void* const* next, double dt, const void* const* dt_elem, double alpha,
                          double beta, const fsg_boundary* boundary, fsg_stage_diag* diag,
                          void* stream) {
-  if (!m || !base || !curr || !total || !work || !next) return fsg_fail(-10, "null argument");
+  if (!m || !base || !curr || !work || !next) return fsg_fail(-10, "null argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   FSG_MOOD_DISPATCH(stage_fused_impl, (m, base, curr, total, visc, work, next, dt, dt_elem,
                                        alpha, beta, boundary, diag, s));

@@ -395,7 +395,8 @@ class HybridMOODSubcellLimiter:
 
     def stage_fused(self, base, curr, total, visc, work, nxt, dt, alpha, beta, boundary_state,
                     diag):
-        """The MOOD branch from one residual pass: ``total`` = rhs(curr);
+        """The MOOD branch from one residual pass: ``total`` = rhs(curr), or
+        None when ``work`` already holds the Euler candidate curr + dt rhs;
         the viscous rhs is evaluated into ``visc`` (or skipped when None) on
         the troubled elements only, after detection."""
         op = self.operator
@@ -408,7 +409,7 @@ class HybridMOODSubcellLimiter:
         bnd = op._boundary(boundary_state)
         nat.check(nat.lib().fsg_mood_stage_fused(
             self._ctx().ctx, nat.ptr_array(op._inputs(base)), nat.ptr_array(op._inputs(curr)),
-            nat.ptr_array([total[t] for t in types]),
+            None if total is None else nat.ptr_array([total[t] for t in types]),
             nat.ptr_array([visc[t] for t in types]) if visc is not None else None,
             nat.ptr_array([work[t] for t in types]), nat.ptr_array([nxt[t] for t in types]),
             float(dt), dt_elem, float(alpha), float(beta), ctypes.byref(bnd), diag,

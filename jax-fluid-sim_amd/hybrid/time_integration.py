@@ -267,17 +267,20 @@ class HybridSSPRK3:
         op = self.operator
         stream = nat.stream_handle(self.solver.device)
         b = op._boundary(boundary_state)
-        Ub, Uc = op._inputs(base), op._inputs(curr)
+        Ub = None if base is None else op._inputs(base)
+        Uc = op._inputs(curr)
         Un = [nxt[t] for t in op.block_types]
         dt_elem = None
         if isinstance(dt, dict):
             dts = [dt[t].reshape(-1).to(self.solver.dtype).contiguous() for t in op.block_types]
             dt_elem = nat.ptr_array(dts)
             dt = 0.0
-        nat.check(nat.lib().fsg_rk_stage(op.device_mesh.ctx, nat.ptr_array(Ub), nat.ptr_array(Uc),
-                                         nat.ptr_array(Un), float(dt), dt_elem, float(alpha),
-                                         float(beta), op.rhs_terms(), ctypes.byref(b),
-                                         nat.ptr(diag.buf), stream))
+        nat.check(nat.lib().fsg_rk_stage(op.device_mesh.ctx,
+                                         None if base is None else nat.ptr_array(Ub),
+                                         nat.ptr_array(Uc), nat.ptr_array(Un), float(dt), dt_elem,
+                                         float(alpha), float(beta), op.rhs_terms(),
+                                         ctypes.byref(b),
+                                         None if diag is None else nat.ptr(diag.buf), stream))
 
     def _mood_stage(self, base, curr, nxt, dt, alpha, beta, boundary_state):
         """The MOOD branch of hybrid/time_integration.py:404-439: inviscid and
@@ -289,10 +292,13 @@ class HybridSSPRK3:
         has_visc = self.solver.dynamic_viscosity > 0.0 or self.solver.use_wale
         viscous = hasattr(op, "inviscid_rhs") and has_visc
         if viscous and self.fused_mood:
-            total = op.rhs(curr, boundary_state)
+            # the residual's epilogue writes the Euler candidate curr + dt rhs
+            # (no base term, no diagnostics) straight into work, which the
+            # detection then reads in place of rhs and curr
             visc = {t: torch.empty_like(curr[t]) for t in op.block_types}
             work = {t: torch.empty_like(curr[t]) for t in op.block_types}
-            self.limiter.stage_fused(base, curr, total, visc, work, nxt, dt, alpha, beta,
+            self._stage_kernel(None, curr, work, dt, 0.0, 1.0, boundary_state, None)
+            self.limiter.stage_fused(base, curr, None, visc, work, nxt, dt, alpha, beta,
                                      boundary_state, nat.ptr(self._diag.buf))
             return
         if hasattr(op, "inviscid_rhs"):
