@@ -8,6 +8,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -156,18 +157,34 @@ __device__ __forceinline__ double element_rate(const real* U, int N, int V, doub
 }
 
 // Global maximum rate: NaN and +inf count as 1e7 (the nan_to_num of
-// :166); non-negative doubles order like their bit patterns.
+// :166); non-negative doubles order like their bit patterns.  A bounded grid
+// strides over the elements (one wave per element at a time), each
+// workgroup reduces its waves' maxima in LDS and issues one atomic: one
+// atomic per element wave on a single address serialised the launch.
+constexpr int RATE_GRID = 4096;
+
 template <typename real>
 __global__ __launch_bounds__(WG) void max_rate_kernel(const real* __restrict__ U, int64_t E,
                                                       int N, int V, const double* __restrict__ L,
                                                       RateParams q, WaleTables<real> w,
                                                       unsigned long long* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const int64_t e = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
-  if (e >= E) return;
-  double r = element_rate<real>(U + e * N * V, N, V, L[e], q, lane, e, w);
-  if (r != r || r > 1.0e300) r = 1.0e7;
-  if (lane == 0) atomicMax(out, (unsigned long long)__double_as_longlong(r));
+  __shared__ unsigned long long sm[WG / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long mb = 0ull;
+  for (int64_t e = (int64_t)blockIdx.x * (WG / 64) + wave; e < E;
+       e += (int64_t)gridDim.x * (WG / 64)) {
+    double r = element_rate<real>(U + e * N * V, N, V, L[e], q, lane, e, w);
+    if (r != r || r > 1.0e300) r = 1.0e7;
+    const unsigned long long rb = (unsigned long long)__double_as_longlong(r);
+    mb = rb > mb ? rb : mb;
+  }
+  if (lane == 0) sm[wave] = mb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 1; k < WG / 64; ++k) mb = sm[k] > mb ? sm[k] : mb;
+    atomicMax(out, mb);
+  }
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p) {
@@ -470,7 +487,7 @@ int fsg_stable_dt(fsg_dt* d, const void* const* U, double cfl, double* dt_out,
     if (!U[b]) return dt_fail(-10, "null state block");
     if (bd.count == 0) continue;
     const int N = nodes_of(bd.element_type, P);
-    const unsigned g = blocks_for(bd.count, fsgdt::WG / 64);
+    const unsigned g = std::min(blocks_for(bd.count, fsgdt::WG / 64), (unsigned)fsgdt::RATE_GRID);
     if (c->params.dtype == FSG_F64)
       hipLaunchKernelGGL(fsgdt::max_rate_kernel<double>, dim3(g), dim3(fsgdt::WG), 0, s,
                          static_cast<const double*>(U[b]), bd.count, N, V, d->blocks[b].length,
