@@ -464,7 +464,8 @@ int fsg_mood_count(fsg_mood* m, int64_t* count, void* stream);
 /* ---- Park 11-species air chemistry (FP64) ---------------------------------
  * table: device buffer of fsg_chem_table_size() doubles holding the
  * mechanism (molar masses, NASA-7 fits, Arrhenius and temperature modes,
- * nu', nu'', third-body efficiencies; layout in fsg_chem.hip), built from
+ * nu', nu'', third-body efficiencies, derived per-reaction and per-species
+ * values, the stoichiometry fingerprint; layout in fsg_chem.hip), built from
  * Air11ParkMechanism (hybrid/chemistry/air11_park.py:20-318). */
 typedef struct {
   int32_t newton_iters;  /* 12 in ReactingFlowCoupler (hybrid/chemistry/coupling.py:30) */
@@ -528,7 +529,10 @@ int fsg_chem_integrate_flagged_noalloc(const double* table, const double* rho_in
  * |R_s| / max(|1 - dt dw_s / drho_s|, 1).  flag[c] is set to 1 where the estimate
  * exceeds rtol |rho_pred,s - rho_0,s| + atol sum(rho_0) for any species (and
  * left as it is elsewhere); err_out (may be null) receives the largest
- * estimate relative to sum(rho_0). */
+ * estimate relative to sum(rho_0).  The rates use the Park-11 stoichiometry
+ * and third-body efficiencies compiled into the library (fsg_park11.h); a
+ * table whose last entry (the stoichiometry fingerprint) differs from the
+ * compiled one is not judged: every cell is flagged (err_out = +inf). */
 int fsg_chem_check(const double* table, const double* rho_in, const double* T_in,
                    const double* dt_cell, double dt, int32_t constant_temperature,
                    const double* rho_pred, double rtol, double atol, int32_t* flag,

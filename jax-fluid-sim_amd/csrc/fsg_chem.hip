@@ -18,6 +18,7 @@
 #include <string>
 
 #include "../../include/fluid_gfx950.h"
+#include "fsg_park11.h"
 
 namespace fsgchem {
 
@@ -42,7 +43,11 @@ constexpr int OFF_ALPHA = OFF_NU2 + NR * NS;
 constexpr int OFF_DERIVED = OFF_ALPHA + NR * NS;
 constexpr int OFF_SPEC = OFF_DERIVED + 3 * NR;
 constexpr int SPEC_W = 18;
-constexpr int TABLE_SIZE = OFF_SPEC + SPEC_W * NS;
+// last entry: the fingerprint of the table's stoichiometry and third-body
+// data (hybrid/chemistry/air11_park.py stoichiometry_fingerprint)
+constexpr int OFF_FP = OFF_SPEC + SPEC_W * NS;
+constexpr int TABLE_SIZE = OFF_FP + 1;
+static_assert(park11::NR == NR && park11::NS == NS, "compiled-in mechanism shape");
 
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
@@ -303,27 +308,27 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
 // speed (values equal to rounding): with T = Tv every temperature mode is
 // tT = max(T, 10); k_f fwd and k_r rev are each one exponential of
 // ln A + b ln tT - theta / tT + lf (and of that minus ln K_c plus lr), the
-// clamps of rates() applied to the same arguments; species that take no
-// part in a reaction are skipped (the stoichiometry is uniform across the
-// wave, so these are scalar branches).
+// clamps of rates() applied to the same arguments.  The stoichiometry and
+// third-body efficiencies are the compiled-in Park-11 constants
+// (fsg_park11.h; the caller checks the table's fingerprint against them),
+// so the reaction loop is unrolled with each reaction's absent species
+// folded away at compile time; the rate constants, thermodynamics and
+// molar masses come from the table.
 __device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const double* rho,
                                            double T, double* w, double* dd) {
   const double* mw = tab;
-  const double* rx = tab + OFF_RX;
-  const double* der = tab + OFF_DERIVED;
-  double C[NS], dC[NS], lc[NS], dlc[NS], g[NS];
+  // C_s and d C_s / d rho_s are formed again where used (not held)
+  double lc[NS], dlc[NS], g[NS];
   const double tT = dmax(T, 10.0);
   const double lT = log(tT), iT = 1.0 / tT;
   const double lP = log(P_STD / (R_UNIV * tT));
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const double* sp = species_block(tab, s);
-    const double x = rho[s] * sp[0];
-    C[s] = dmax(x, 0.0);
-    dC[s] = dmax_dx(x, 0.0) * sp[0];
-    double cs = dmax(C[s], 1.0e-100);
+    const double C = dmax(rho[s] * sp[0], 0.0);
+    const double cs = dmax(C, 1.0e-100);
     lc[s] = log(cs);
-    dlc[s] = dmax_dx(C[s], 1.0e-100) / cs;
+    dlc[s] = dmax_dx(C, 1.0e-100) / cs;
     g[s] = gibbs(tab + OFF_NASA + 17 * s, sp, tT, lT, iT);
   }
   // the diagonal only scales the error estimate: its accumulators and
@@ -335,23 +340,24 @@ __device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const
     om[s] = 0.0;
     dom[s] = csum[s] = 0.0f;
   }
+#pragma unroll
   for (int r = 0; r < NR; ++r) {
-    const double b = rx[6 * r + 1], th = rx[6 * r + 2];
-    const bool third = rx[6 * r + 3] != 0.0;
-    const double* n1 = tab + OFF_NU1 + r * NS;
-    const double* n2 = tab + OFF_NU2 + r * NS;
-    const double* al = tab + OFF_ALPHA + r * NS;
-    const double lnA = der[3 * r + 0], dn = der[3 * r + 1], mr = der[3 * r + 2];
+    const double* tb = table_launder(tab);
+    const double b = tb[OFF_RX + 6 * r + 1], th = tb[OFF_RX + 6 * r + 2];
+    const bool third = park11::THIRD[r];
+    const double lnA = tb[OFF_DERIVED + 3 * r + 0], dn = tb[OFF_DERIVED + 3 * r + 1];
+    const double mr = tb[OFF_DERIVED + 3 * r + 2];
     double dG = 0.0, lf = 0.0, lr = 0.0, Meff = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double a1 = n1[s], a2 = n2[s];
+      const double a1 = park11::NU1[r][s], a2 = park11::NU2[r][s];
       if (a1 != 0.0 || a2 != 0.0) {
         dG = fma(a2 - a1, g[s], dG);
         lf = fma(a1, lc[s], lf);
         lr = fma(a2, lc[s], lr);
       }
-      if (third && al[s] != 0.0) Meff = fma(al[s], C[s], Meff);
+      if (third && park11::ALPHA[r][s] != 0.0)
+        Meff = fma(park11::ALPHA[r][s], dmax(rho[s] * tb[OFF_SPEC + SPEC_W * s], 0.0), Meff);
     }
     const double lkf = fma(-th, iT, fma(b, lT, lnA));
     // ln K_c = -clamp(dG) + dn ln(P / R T); k_r = k_f / max(K_c, 1e-300)
@@ -364,11 +370,13 @@ __device__ __forceinline__ void rates_diag(const double* __restrict__ tab, const
     const double q = (fwdk - revk) * col;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double a1 = n1[s], a2 = n2[s];
-      const double dcol = third ? al[s] : 0.0;
+      const double a1 = park11::NU1[r][s], a2 = park11::NU2[r][s];
+      const double dcol = third ? park11::ALPHA[r][s] : 0.0;
       if (a1 != 0.0 || a2 != 0.0 || dcol != 0.0) {
+        const double m = tb[OFF_SPEC + SPEC_W * s];
+        const double dC = dmax_dx(rho[s] * m, 0.0) * m;
         const float dq = (float)(fma(col * fma(fwdk * cf, a1, -(revk * cr * a2)), dlc[s],
-                                     dcol * (fwdk - revk)) * dC[s]);
+                                     dcol * (fwdk - revk)) * dC);
         const double d = a2 - a1;
         om[s] = fma(d, q, om[s]);
         dom[s] = fmaf((float)d, dq, dom[s]);
@@ -636,6 +644,14 @@ __global__ __launch_bounds__(64) void check_kernel(const double* __restrict__ ta
                                                    double* __restrict__ err_out, int64_t n) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  // the rates use the compiled-in Park-11 stoichiometry: a table built for
+  // other stoichiometry or third-body data is not judged here, its cells go
+  // to the integrator
+  if (tab[OFF_FP] != park11::FINGERPRINT) {
+    flag[c] = 1;
+    if (err_out) err_out[c] = __builtin_huge_val();
+    return;
+  }
   // rho_0 is read again after the rates rather than held across them
   double rp[NS], Y0[NS], total = 0.0;
 #pragma unroll

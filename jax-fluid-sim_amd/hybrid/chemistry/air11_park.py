@@ -83,23 +83,49 @@ _REACTIONS = (
 )
 
 
+def stoichiometry():
+    """(nu', nu'', third-body efficiencies, third-body flags) as (22, 11)
+    float64 arrays and a bool vector."""
+    ns, nr = 11, len(_REACTIONS)
+    nu1, nu2, alpha = np.zeros((nr, ns)), np.zeros((nr, ns)), np.zeros((nr, ns))
+    third = np.zeros(nr, bool)
+    for r, (rea, pro, A, b, th, t3, eff, fm, rm) in enumerate(_REACTIONS):
+        for s, c in rea.items():
+            nu1[r, s] += c
+        for s, c in pro.items():
+            nu2[r, s] += c
+        third[r] = bool(t3)
+        if t3:
+            alpha[r, 1:] = 1.0          # every species but the electron
+            for s, e in eff.items():
+                alpha[r, s] = e
+    return nu1, nu2, alpha, third
+
+
+def stoichiometry_fingerprint(nu1, nu2, alpha, third):
+    """One float64 that identifies the stoichiometry and third-body data: the
+    kernels that compile these in (csrc/fsg_park11.h) compare it with the
+    table's last entry.  Summed in a fixed order, so it is reproducible."""
+    nr, ns = nu1.shape
+    fp = 0.0
+    for r in range(nr):
+        fp += 1000.0 * (r + 1) * float(bool(third[r]))
+        for s in range(ns):
+            fp += (r * ns + s + 1) * (float(nu1[r, s]) + 2.0 * float(nu2[r, s])
+                                      + 4.0 * float(alpha[r, s]))
+    return fp
+
+
 def mechanism_table():
     """Flat float64 table in the layout of csrc/fsg_chem.hip."""
     ns, nr = 11, len(_REACTIONS)
     nasa = []
     for lo, mid, hi, cl, ch in _NASA:
         nasa += [lo, mid, hi, *cl, *ch]
-    rx, nu1, nu2, alpha = [], np.zeros((nr, ns)), np.zeros((nr, ns)), np.zeros((nr, ns))
+    rx = []
     for r, (rea, pro, A, b, th, third, eff, fm, rm) in enumerate(_REACTIONS):
         rx += [A, b, th, float(third), float(fm), float(rm)]
-        for s, c in rea.items():
-            nu1[r, s] += c
-        for s, c in pro.items():
-            nu2[r, s] += c
-        if third:
-            alpha[r, 1:] = 1.0          # every species but the electron
-            for s, e in eff.items():
-                alpha[r, s] = e
+    nu1, nu2, alpha, third_flags = stoichiometry()
     # derived per reaction: ln A, sum(nu'' - nu'), sum M (nu'' - nu')
     d = nu2 - nu1
     derived = np.stack([np.log([r[2] for r in _REACTIONS]), d.sum(1),
@@ -113,5 +139,7 @@ def mechanism_table():
         for a in (cl, ch):
             spec += [a[1] / 2.0, a[2] / 3.0, a[3] / 4.0, a[4] / 5.0, a[2] / 2.0, a[3] / 3.0,
                      a[4] / 4.0]
+    # last: the stoichiometry fingerprint (csrc/fsg_park11.h)
+    fp = stoichiometry_fingerprint(nu1, nu2, alpha, third_flags)
     return np.concatenate([np.asarray(MOLAR_MASSES), nasa, rx, nu1.ravel(), nu2.ravel(),
-                           alpha.ravel(), derived.ravel(), spec]).astype(np.float64)
+                           alpha.ravel(), derived.ravel(), spec, [fp]]).astype(np.float64)

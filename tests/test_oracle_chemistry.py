@@ -169,7 +169,7 @@ def test_device_table_transcription_matches_oracle():
     # coefficients per fit range; the Horner forms they feed equal the power
     # sums of the fits
     sp = tab[o:o + 18 * ns].reshape(ns, 18)
-    assert o + 18 * ns == len(tab)
+    assert o + 18 * ns + 1 == len(tab)   # then the stoichiometry fingerprint
     assert np.allclose(sp[:, 0] * MOLAR_MASS, 1.0, rtol=1e-15, atol=0)
     assert np.allclose(sp[:, 1] * MOLAR_MASS, R_UNIV, rtol=1e-15, atol=0)
     for s, (lo, mid, hi, cl, ch) in enumerate(MECH.nasa):

@@ -339,9 +339,9 @@ def test_product_mechanism_table_matches_reference_data():
                            for f in (sp["nasa7"] for sp in g["species"])])
     want = np.concatenate([[sp["molar_mass"] for sp in g["species"]], nasa, rx.ravel(),
                            nu1.ravel(), nu2.ravel(), alpha.ravel()])
-    # the reference data, then 3 derived values per reaction and 18 per
-    # species (pinned in test_oracle_chemistry)
-    assert len(tab) == len(want) + 3 * nr + 18 * ns and np.array_equal(tab[:len(want)], want)
+    # the reference data, then 3 derived values per reaction, 18 per species
+    # (pinned in test_oracle_chemistry) and the stoichiometry fingerprint
+    assert len(tab) == len(want) + 3 * nr + 18 * ns + 1 and np.array_equal(tab[:len(want)], want)
     assert ns == 11 and nr == 22
 
 
