@@ -193,7 +193,12 @@ __device__ __forceinline__ double recover_temperature_converged(const double* ta
 // skipped (the stoichiometry is uniform across the wave: scalar branches);
 // the Jacobian's column sums come from the reactions' mass balances sum_s
 // M_s (nu''_s - nu'_s); the mass-sum correction w - Y sum(w) is
-// differentiated with one reciprocal of the total.
+// differentiated with one reciprocal of the total.  PARK: the stoichiometry
+// and third-body data are the compiled-in Park-11 constants (fsg_park11.h)
+// and the reaction loop is unrolled, each reaction's absent species folded
+// away at compile time (same arithmetic); otherwise they are read from the
+// table, one reaction per loop iteration.
+template <bool PARK>
 __device__ __forceinline__ void rates(const double* __restrict__ tab, const double* rho, double T,
                                       double* w, double (*dw)[NS]) {
   const double* mw = tab;
@@ -221,28 +226,30 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
 #pragma unroll
     for (int j = 0; j < NS; ++j) dw[s][j] = 0.0;
   }
-#pragma unroll 1
-  for (int r = 0; r < NR; ++r) {
+  auto reaction = [&](int r) {
     const double* tb = table_launder(tab);
     const double* rx = tb + OFF_RX + 6 * r;
     const double b = rx[1], th = rx[2];
-    const bool third = rx[3] != 0.0;
+    const bool third = PARK ? park11::THIRD[r] : rx[3] != 0.0;
     const double* n1 = tb + OFF_NU1 + r * NS;
     const double* n2 = tb + OFF_NU2 + r * NS;
-    const double* al = tb + OFF_ALPHA + r * NS;
+    const double* alt = tb + OFF_ALPHA + r * NS;
+    auto nu1 = [&](int k) { return PARK ? park11::NU1[r][k] : n1[k]; };
+    auto nu2 = [&](int k) { return PARK ? park11::NU2[r][k] : n2[k]; };
+    auto al = [&](int k) { return PARK ? park11::ALPHA[r][k] : alt[k]; };
     const double* dr = tb + OFF_DERIVED + 3 * r;
     const double lnA = dr[0], dn = dr[1], mr = dr[2];
     double dG = 0.0, lf = 0.0, lr = 0.0, Meff = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double a1 = n1[s], a2 = n2[s];
+      const double a1 = nu1(s), a2 = nu2(s);
       if (a1 != 0.0 || a2 != 0.0) {
         dG = fma(a2 - a1, g[s], dG);
         lf = fma(a1, lc[s], lf);
         lr = fma(a2, lc[s], lr);
       }
-      if (third && al[s] != 0.0)
-        Meff = fma(al[s], dmax(rho[s] * tb[OFF_SPEC + SPEC_W * s], 0.0), Meff);
+      if (third && al(s) != 0.0)
+        Meff = fma(al(s), dmax(rho[s] * tb[OFF_SPEC + SPEC_W * s], 0.0), Meff);
     }
     const double lkf = fma(-th, iT, fma(b, lT, lnA));
     const double lKc = dmax(fma(dn, lP, -clampd(dG, -700.0, 700.0)), -690.7755278982137);
@@ -256,25 +263,32 @@ __device__ __forceinline__ void rates(const double* __restrict__ tab, const doub
     const double fc = col * fwdk * cf, rc = col * revk * cr;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double d = n2[s] - n1[s];
+      const double d = nu2(s) - nu1(s);
       if (d != 0.0) om[s] = fma(d, q, om[s]);
     }
     // column j of this reaction's d q / d rho, added to every row it feeds
     // (no per-reaction vector held)
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const double a1 = n1[j], a2 = n2[j];
-      const double dcol = third ? al[j] : 0.0;
+      const double a1 = nu1(j), a2 = nu2(j);
+      const double dcol = third ? al(j) : 0.0;
       if (a1 != 0.0 || a2 != 0.0 || dcol != 0.0) {
         const double dq = fma(fma(fc, a1, -(rc * a2)), dlc[j], dcol * net) * dC[j];
         csum[j] = fma(mr, dq, csum[j]);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-          const double d = n2[s] - n1[s];
+          const double d = nu2(s) - nu1(s);
           if (d != 0.0) dw[s][j] = fma(tb[s] * d, dq, dw[s][j]);
         }
       }
     }
+  };
+  if constexpr (PARK) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) reaction(r);
+  } else {
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) reaction(r);
   }
   // w = M omega, then the exact mass-sum correction w - Y sum(w)
   double srho = 0.0, S = 0.0;
@@ -454,6 +468,7 @@ __device__ __forceinline__ void solve(double (*A)[NS], double* b) {
 // a fixed Newton count, non-negativity clip, temperature recovery at the
 // initial internal energy after each iteration (unless held), total-mass
 // rescale.  The reference evaluates its rates at (T, Tv) with Tv = T here.
+template <bool PARK>
 __device__ __forceinline__ double integrate(const double* tab, double* rho, double T, double dt, int iters,
                             int constant_temperature) {
   double rho0[NS], Y0[NS];
@@ -476,7 +491,7 @@ __device__ __forceinline__ double integrate(const double* tab, double* rho, doub
   for (int it = 0; it < iters; ++it) {
     double w[NS];
     double J[NS][NS];
-    rates(tab, r, Tk, w, J);
+    rates<PARK>(tab, r, Tk, w, J);
     double rhs[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -497,6 +512,15 @@ __device__ __forceinline__ double integrate(const double* tab, double* rho, doub
   return Tk;
 }
 
+// The integrator kernels come in two instantiations, launched together: PARK
+// (the compiled-in Park-11 stoichiometry) and the table-driven one; each
+// leaves at once unless the table's stoichiometry fingerprint selects it.
+template <bool PARK>
+__device__ __forceinline__ bool selected(const double* tab) {
+  return (tab[OFF_FP] == park11::FINGERPRINT) == PARK;
+}
+
+template <bool PARK>
 __global__ __launch_bounds__(64) void integrate_kernel(const double* __restrict__ tab,
                                                        const double* __restrict__ rho_in,
                                                        const double* __restrict__ T_in,
@@ -505,25 +529,26 @@ __global__ __launch_bounds__(64) void integrate_kernel(const double* __restrict_
                                                        double* __restrict__ rho_out,
                                                        double* __restrict__ T_out, int64_t n) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+  if (c >= n || !selected<PARK>(tab)) return;
   double r[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) r[s] = rho_in[c * NS + s];
   double h = dt_cell ? dt_cell[c] : dt;
-  double T = integrate(tab, r, T_in[c], h, iters, const_T);
+  double T = integrate<PARK>(tab, r, T_in[c], h, iters, const_T);
 #pragma unroll
   for (int s = 0; s < NS; ++s) rho_out[c * NS + s] = r[s];
   if (T_out) T_out[c] = T;
 }
 
 // hybrid/chemistry/coupling.py:116-177 on a (cells, 16) state, in place.
+template <bool PARK>
 __global__ __launch_bounds__(64) void react_kernel(const double* __restrict__ tab,
                                                    double* __restrict__ U, int64_t n,
                                                    int64_t nodes_per_elem,
                                                    const double* __restrict__ dt_elem, double dt,
                                                    fsg_chem_params p) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+  if (c >= n || !selected<PARK>(tab)) return;
   double* u = U + c * (5 + NS);
   double h = dt_elem ? dt_elem[c / nodes_per_elem] : dt;
   double rt = dmax(u[0], 1.0e-14);
@@ -548,7 +573,7 @@ __global__ __launch_bounds__(64) void react_kernel(const double* __restrict__ ta
   }
   const double Tg = dl ? dmax((p.gamma - 1.0) * eth_s * p.T_ref, 200.0) : 1000.0;
   const double T0 = recover_temperature_converged(tab, rs, eth, Tg, 8);
-  integrate(tab, rs, T0, dl ? h * p.t_ref : h, p.newton_iters, 0);
+  integrate<PARK>(tab, rs, T0, dl ? h * p.t_ref : h, p.newton_iters, 0);
   if (dl) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) rs[s] = rs[s] / p.rho_ref;
@@ -713,11 +738,13 @@ __global__ __launch_bounds__(256) void compact_flags_kernel(const int32_t* __res
   }
 }
 
+template <bool PARK>
 __global__ __launch_bounds__(64) void integrate_list_kernel(
     const double* __restrict__ tab, const double* __restrict__ rho_in,
     const double* __restrict__ T_in, const double* __restrict__ dt_cell, double dt, int iters,
     int const_T, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
     double* __restrict__ rho_out, double* __restrict__ T_out) {
+  if (!selected<PARK>(tab)) return;
   const int64_t m = *count;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -726,7 +753,7 @@ __global__ __launch_bounds__(64) void integrate_list_kernel(
 #pragma unroll
     for (int s = 0; s < NS; ++s) r[s] = rho_in[c * NS + s];
     const double h = dt_cell ? dt_cell[c] : dt;
-    const double T = integrate(tab, r, T_in[c], h, iters, const_T);
+    const double T = integrate<PARK>(tab, r, T_in[c], h, iters, const_T);
 #pragma unroll
     for (int s = 0; s < NS; ++s) rho_out[c * NS + s] = r[s];
     if (T_out) T_out[c] = T;
@@ -760,7 +787,10 @@ int fsg_chem_integrate(const double* table, const double* rho_in, const double* 
   if (n <= 0) return 0;
   if (newton_iters < 0) return chem_fail(-11, "negative iteration count");
   unsigned grid = (unsigned)((n + 63) / 64);
-  hipLaunchKernelGGL(fsgchem::integrate_kernel, dim3(grid), dim3(64), 0,
+  hipLaunchKernelGGL(fsgchem::integrate_kernel<true>, dim3(grid), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), table, rho_in, T_in, dt_cell, dt,
+                     newton_iters, constant_temperature, rho_out, T_out, n);
+  hipLaunchKernelGGL(fsgchem::integrate_kernel<false>, dim3(grid), dim3(64), 0,
                      static_cast<hipStream_t>(stream), table, rho_in, T_in, dt_cell, dt,
                      newton_iters, constant_temperature, rho_out, T_out, n);
   return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
@@ -773,7 +803,10 @@ int fsg_chem_react(const double* table, double* U, int64_t ncells, int64_t nodes
   if (ncells <= 0) return 0;
   if (nodes_per_elem <= 0) return chem_fail(-12, "nodes_per_elem must be positive");
   unsigned grid = (unsigned)((ncells + 63) / 64);
-  hipLaunchKernelGGL(fsgchem::react_kernel, dim3(grid), dim3(64), 0,
+  hipLaunchKernelGGL(fsgchem::react_kernel<true>, dim3(grid), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), table, U, ncells, nodes_per_elem, dt_elem,
+                     dt, *params);
+  hipLaunchKernelGGL(fsgchem::react_kernel<false>, dim3(grid), dim3(64), 0,
                      static_cast<hipStream_t>(stream), table, U, ncells, nodes_per_elem, dt_elem,
                      dt, *params);
   return hipGetLastError() == hipSuccess ? 0 : chem_fail(-1, "chemistry launch failed");
@@ -797,7 +830,11 @@ int fsg_chem_integrate_flagged(const double* table, const double* rho_in, const 
   // integrator's resident capacity (one wave per SIMD at its register count)
   // and leave at once when nothing is flagged
   const int64_t blocks = (n + 63) / 64 < 8192 ? (n + 63) / 64 : 8192;
-  hipLaunchKernelGGL(fsgchem::integrate_list_kernel, dim3((unsigned)blocks), dim3(64),
+  hipLaunchKernelGGL(fsgchem::integrate_list_kernel<true>, dim3((unsigned)blocks), dim3(64),
+                     0, s, table,
+                     rho_in, T_in, dt_cell, dt, newton_iters, constant_temperature, list, count,
+                     rho_out, T_out);
+  hipLaunchKernelGGL(fsgchem::integrate_list_kernel<false>, dim3((unsigned)blocks), dim3(64),
                      0, s, table,
                      rho_in, T_in, dt_cell, dt, newton_iters, constant_temperature, list, count,
                      rho_out, T_out);
