@@ -914,11 +914,18 @@ template <int V, int KS, typename real>
 __device__ void positivity_wave(const Args<real>& a, const Phys<real>& ph, const TypeT<real>& T,
                                 real (*sU)[V], real* sM, const real* sW, int lane) {
   const int N = T.N, NT = N + T.NS;
-  if (lane < V) {
+  {
+    // the mean: lane l sums variable l % VP over the nodes j = l / VP (mod
+    // 64 / VP), then the partial sums of a variable are added across lanes
+    constexpr int VP = V <= 8 ? 8 : 16, G = 64 / VP;
+    static_assert(V <= VP, "variables per lane group");
+    const int v = lane % VP, part = lane / VP;
     real m = real(0);
-#pragma unroll 8
-    for (int j = 0; j < N; ++j) m += sW[j] * sU[j][lane];
-    sM[lane] = m / T.wsum;
+    if (v < V)
+      for (int j = part; j < N; j += G) m += sW[j] * sU[j][v];
+#pragma unroll
+    for (int o = VP; o < 64; o <<= 1) m += __shfl_xor(m, o, 64);
+    if (part == 0 && v < V) sM[v] = m / T.wsum;
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1047,7 +1054,7 @@ constexpr int MAXNT = 64 + 640;   // nodes + sample points of one element
 // mode 0: out = positivity(src); mode 1 (stage): out = positivity(alpha base
 // + beta positivity(work)) with the node admissibility reductions.
 template <int V, int KS, typename real>
-__global__ __launch_bounds__(64) void positivity_kernel(Args<real> a, Phys<real> ph, int mode,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && KS <= 5 ? 8 : 1))) void positivity_kernel(Args<real> a, Phys<real> ph, int mode,
                                                         int64_t g0) {
   __shared__ real sU[MAXN][V];
   __shared__ real sM[V];
