@@ -478,6 +478,10 @@ typedef struct {
 } fsg_chem_params;
 
 int32_t fsg_chem_table_size(void);
+/* The integrator entry points below use the Park-11 stoichiometry and
+ * third-body data compiled into the library (fsg_park11.h) when the table's
+ * last entry (the stoichiometry fingerprint) matches them, and read them
+ * from the table otherwise (same results). */
 
 /* PointImplicitIntegrator.step over n cells
  * (hybrid/chemistry/integrator.py:31-104): rho_in/rho_out (n, 11), T_in (n),
