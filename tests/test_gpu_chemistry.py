@@ -270,3 +270,53 @@ def test_integrator_check_matches_oracle(const_T):
     print(f"check: flagged {want_f.mean():.2f}, compared {margin.mean():.2f}", flush=True)
     assert margin.mean() > 0.9 and 0 < want_f.mean() < 1
     assert np.array_equal(got_f[margin], want_f[margin])
+
+
+def _other_fingerprint(mech):
+    """The same mechanism with another stoichiometry fingerprint: the
+    kernels then take the table-driven instantiations."""
+    other = Air11ParkMechanism()
+    other.table = mech.table.clone()
+    other.table[-1] = 0.0
+    return other
+
+
+@pytest.mark.parametrize("const_T,iters", [(True, 4), (False, 3)])
+def test_table_driven_integrator_equals_compiled_stoichiometry(const_T, iters):
+    """The integrator with the Park-11 stoichiometry compiled in and the
+    table-driven one compute the same arithmetic: bit-identical results."""
+    mech = Air11ParkMechanism()
+    other = _other_fingerprint(mech)
+    rho, T = make_batch(4096, seed=5)
+    a = PointImplicitIntegrator(mech, max_newton_iters=iters).step(rho, T, 1e-7, constant_temperature=const_T)
+    b = PointImplicitIntegrator(other, max_newton_iters=iters).step(rho, T, 1e-7, constant_temperature=const_T)
+    assert torch.isfinite(a[0]).all()
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    U = torch.as_tensor(_random_block(True, seed=11), device="cuda")
+    ua = ReactingFlowCoupler(mech, max_newton_iters=iters, dimensionless=True).react_block(U, 1e-6)
+    ub = ReactingFlowCoupler(other, max_newton_iters=iters, dimensionless=True).react_block(U, 1e-6)
+    assert torch.equal(ua, ub)
+
+
+def test_check_refuses_a_table_it_was_not_compiled_for():
+    """fsg_chem_check judges with the compiled-in stoichiometry only: a table
+    whose fingerprint differs has every cell flagged for the integrator."""
+    import ctypes
+    from hybrid import _native as nat
+    mech = Air11ParkMechanism()
+    other = _other_fingerprint(mech)
+    rho, T = air_batch(256, seed=4, T_range=(1500.0, 3000.0))
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    Rd, Td = dev(rho), dev(T)
+    for table, want in ((mech.table, False), (other.table, True)):
+        flag = torch.zeros(256, dtype=torch.int32, device="cuda")
+        err = torch.empty(256, dtype=torch.float64, device="cuda")
+        # the prediction "no change" over a 1e-20 s step passes the check
+        nat.check_chem(nat.lib().fsg_chem_check(nat.ptr(table), nat.ptr(Rd), nat.ptr(Td), None,
+                                                1e-20, 1, nat.ptr(Rd), 0.1, 1e-8, nat.ptr(flag),
+                                                nat.ptr(err), 256, nat.stream_handle(Rd.device)))
+        f, e = flag.cpu().numpy().astype(bool), err.cpu().numpy()
+        if want:
+            assert f.all() and np.isinf(e).all()
+        else:
+            assert not f.any() and np.isfinite(e).all()
