@@ -367,9 +367,12 @@ __device__ void dg_point_flux(const Args<real>& a, const Phys<real>& ph, const G
 // ---------------------------------------------------------------------------
 // Detection (hybrid/limiter.py:150-185, 285-300).  stage != 0: the detected
 // state is alpha base + beta (curr + dt (inv + visc)), and curr + dt (inv +
-// visc) is written to work; otherwise the detected state is det_src.
+// visc) is written to work; otherwise the detected state is det_src.  FP32
+// instantiations are held to six waves per SIMD (a few bytes of register
+// spill per lane; the extra waves keep more element loads in flight: 1.27 ->
+// 1.00 ms per reacting-mesh launch).
 template <int V, typename real, int EPW>
-__global__ __launch_bounds__(256) void detect_kernel(Args<real> a, Phys<real> ph, int64_t total) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 ? 6 : 1))) void detect_kernel(Args<real> a, Phys<real> ph, int64_t total) {
   // EPW elements per wave, LG = 64 / EPW lanes each (EPW = 2 needs N <= 32)
   constexpr int LG = 64 / EPW, NE = MAXN / EPW;
   __shared__ real sU[4 * EPW][NE][V];
