@@ -313,6 +313,39 @@ def test_fused_stage_matches_split_stage(dtype):
     assert np.all(rel_l2(out[True], out[False]) <= tol), rel_l2(out[True], out[False])
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_fused_stage_from_total_matches_euler_candidate(dtype):
+    """fsg_mood_stage_fused from the total rhs (detection forms curr + dt rhs
+    itself) against the form the stepper uses (the residual's epilogue
+    writes that Euler candidate, no total passed): the same troubled
+    elements, states equal to rounding, the same diagnostics."""
+    from hybrid import _native as nat
+    solver, op, b, lim, oop, ob, olim, states = _case("split", 2, dtype=dtype)
+    curr = {t: oop.solver.primitive_to_conserved(shock_primitive(x, seed=9))
+            for t, x in oop.geometry.solution_node_coordinates.items()}
+    S, C = _dev(solver, states), _dev(solver, curr)
+    drk = HybridSSPRK3(op, limiter=lim)
+    drk.fused_mood = True
+    got, valid, rmin, pmin = drk.rk_stage(S, C, 2e-3, 0.75, 0.25, b)
+    flags = {t: lim._ctx().tables.flags[k].clone() for k, t in enumerate(op.block_types)}
+    stream = nat.stream_handle(solver.device)
+    drk._diag.reset(stream)
+    total = op.rhs(C, b)
+    visc = {t: torch.empty_like(C[t]) for t in op.block_types}
+    work = {t: torch.empty_like(C[t]) for t in op.block_types}
+    nxt = {t: torch.empty_like(C[t]) for t in op.block_types}
+    lim.stage_fused(S, C, total, visc, work, nxt, 2e-3, 0.75, 0.25, b, nat.ptr(drk._diag.buf))
+    valid2, rmin2, pmin2 = drk._diag.values(drk.density_floor, drk.pressure_floor)
+    for k, t in enumerate(op.block_types):
+        assert torch.equal(lim._ctx().tables.flags[k], flags[t]), t
+    assert sum(int(f.sum()) for f in flags.values()) > 0
+    tol = 1e-13 if dtype == torch.float64 else 1e-5
+    assert np.all(rel_l2(_np(nxt), _np(got)) <= tol), rel_l2(_np(nxt), _np(got))
+    assert bool(valid) == bool(valid2)
+    assert abs(float(rmin) - float(rmin2)) <= tol * abs(float(rmin))
+    assert abs(float(pmin) - float(pmin2)) <= 1e3 * tol * abs(float(pmin))
+
+
 @pytest.mark.parametrize("name,p", [("split", 2), ("split", 3), ("sliver", 2)])
 def test_modal_shock_limiter_matches(name, p):
     """HybridModalShockLimiter (hybrid/limiter.py:186-264) against the oracle,
