@@ -201,6 +201,30 @@ def test_in_place_stage_matches_out_of_place():
         assert torch.equal(a[k], V[k])
 
 
+@pytest.mark.parametrize("name,p", [("split", 2), ("species", 2), ("sliver", 2)])
+def test_euler_candidate_stage_without_base(name, p):
+    """fsg_rk_stage with no base state and no diagnostics (the Euler
+    candidate curr + dt rhs(curr) a MOOD stage starts from) equals the same
+    stage with the base weight 0, bit for bit, and the oracle's candidate to
+    the FP64 bar; the element paths (sum-factorised hexahedra, dense wedges
+    and tetrahedra) all write it."""
+    from hybrid import _native as nat
+    ds, dop, db, oop, ob, states = _case(name, p)
+    U = ds.to_device(states)
+    dt = 1e-4 if name == "sliver" else 1e-3
+    dint = HybridSSPRK3(dop)
+    cand = {k: torch.empty_like(v) for k, v in U.items()}
+    dint._stage_kernel(None, U, cand, dt, 0.0, 1.0, db, None)
+    with_base = {k: torch.empty_like(v) for k, v in U.items()}
+    dint._diag.reset(nat.stream_handle(ds.device))
+    dint._stage_kernel(U, U, with_base, dt, 0.0, 1.0, db, dint._diag)
+    for k in U:
+        assert torch.equal(cand[k], with_base[k]), k
+    rhs = oop.rhs(states, ob)
+    ref = {k: states[k] + dt * rhs[k] for k in states}
+    assert np.all(rel_l2(_np(cand), ref) <= FP64_TOL)
+
+
 @pytest.mark.parametrize("name,p", [("smoke", 2), ("split", 3), ("tgv", 3), ("species", 2),
                                     ("sliver", 2), ("sliver", 3)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
