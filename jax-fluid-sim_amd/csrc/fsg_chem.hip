@@ -597,7 +597,8 @@ __global__ __launch_bounds__(64) void react_kernel(const double* __restrict__ ta
 // gains on the latency of the temperature recovery (683 -> 562 us per launch
 // on the reacting mesh's blocks).
 template <typename ureal>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void prepare_kernel(const double* __restrict__ tab,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+void prepare_kernel(const double* __restrict__ tab,
                                                      const ureal* __restrict__ U, int64_t n,
                                                      fsg_chem_params p, double* __restrict__ rho_out,
                                                      double* __restrict__ T_out) {
@@ -664,7 +665,8 @@ __global__ __launch_bounds__(64) void finish_kernel(ureal* __restrict__ U, int64
 // flagged (flag |= 1) for the integrator.  Held to three waves per SIMD:
 // the 216 B of register spill per lane cost less than the third wave gains
 // (1.75 -> 1.66 ms per reacting-mesh launch).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void check_kernel(const double* __restrict__ tab,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+void check_kernel(const double* __restrict__ tab,
                                                    const double* __restrict__ rho_in,
                                                    const double* __restrict__ T_in,
                                                    const double* __restrict__ dt_cell, double dt,

@@ -372,7 +372,8 @@ __device__ void dg_point_flux(const Args<real>& a, const Phys<real>& ph, const G
 // spill per lane; the extra waves keep more element loads in flight: 1.27 ->
 // 1.00 ms per reacting-mesh launch).
 template <int V, typename real, int EPW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 ? 6 : 1))) void detect_kernel(Args<real> a, Phys<real> ph, int64_t total) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 ? 6 : 1)))
+void detect_kernel(Args<real> a, Phys<real> ph, int64_t total) {
   // EPW elements per wave, LG = 64 / EPW lanes each (EPW = 2 needs N <= 32)
   constexpr int LG = 64 / EPW, NE = MAXN / EPW;
   __shared__ real sU[4 * EPW][NE][V];
@@ -1057,8 +1058,9 @@ constexpr int MAXNT = 64 + 640;   // nodes + sample points of one element
 // mode 0: out = positivity(src); mode 1 (stage): out = positivity(alpha base
 // + beta positivity(work)) with the node admissibility reductions.
 template <int V, int KS, typename real>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && KS <= 5 ? 8 : 1))) void positivity_kernel(Args<real> a, Phys<real> ph, int mode,
-                                                        int64_t g0) {
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && KS <= 5 ? 8 : (sizeof(real) == 4 && KS == 8 ? 6 : 1))))
+void positivity_kernel(Args<real> a, Phys<real> ph, int mode, int64_t g0) {
   __shared__ real sU[MAXN][V];
   __shared__ real sM[V];
   __shared__ real sW[MAXN];
